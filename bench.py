@@ -1,0 +1,191 @@
+"""Throughput of the gfx950 Shack-Hartmann engine: subaperture images + centroids per second.
+
+Default workload: the 80x80 WFS at nfft=32 (D=39 m, 0.28 obstruction, 1280x1280 phase, n=16,
+nimg=16, noise off), 64 frames of synthetic von Karman phase per step, resident in HBM before the
+timed region. One step = one device-resident engine call over the 64 frames. Prints one JSON line.
+
+python bench.py [--gpus 1] [--steps K] [--warmup W] [--form fft32|bf16|fp16] [--workload c3|c2|c5|c1]
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+MFMA_PEAK_TFLOPS = {"bf16": 2500.0, "fp16": 2500.0}
+
+
+def algorithmic_bytes(w, nactive, frames, images, noise):
+    """HBM bytes one launch must move: phase windows in, slopes out (+ images, + pre-drawn noise),
+    plus the per-subaperture pupil read once per launch."""
+    per = 4 * w.n * w.n + 8
+    if images:
+        per += 4 * w.nimg * w.nimg
+    if noise:
+        per += 2 * 4 * w.nimg * w.nimg
+    return frames * nactive * per + nactive * 4 * w.n * w.n
+
+
+def cpu_baseline(w, phs0, budget_s=10.0):
+    """CPU leg: the FP32 C restatement of the reference engine (oracle/cref), threaded like the
+    reference's pthread pool over this host's cores (at most 16), timed on repeated frames of the
+    same workload until the budget is spent; then the FP64 oracle on frame 0 for the error figure.
+    Returns (subap-images/s, threads, sample description, FP64 slopes of frame 0)."""
+    from oracle import centoracle as O
+    from oracle.cref import CRef
+    from dasp_amd.util.tel import obstructed_pupil
+    pup = obstructed_pupil(w.nsubx, w.n, w.obs, 0.5)
+    pupfn = pup.perSubap(w.nsubx, vectorAlign=0)
+    flag = pup.getSubapFlag(w.nsubx, 0.5)
+    frac = (pup.getSubarea(w.nsubx) / (w.n * w.n)).astype(np.float32)
+    nact = int(flag.sum())
+    nthr = max(1, min(16, os.cpu_count() or 1))
+    cr = CRef(w.nsubx, w.n, w.nfft, w.clip, w.nimg, w.cen, pupfn.reshape(-1, w.n, w.n), flag, frac, sig=float(w.sig))
+    cr.run(phs0, nthreads=nthr)
+    t0 = time.perf_counter()
+    frames = 0
+    while True:
+        cr.run(phs0, nthreads=nthr)
+        frames += 1
+        if time.perf_counter() - t0 > budget_s:
+            break
+    dt = time.perf_counter() - t0
+    p = O.Params(w.nsubx, w.n, w.nfft, w.clip, w.nimg, w.cen, pupfn, flag, frac, sig=float(w.sig))
+    oc, _, _ = O.run_phase(p, phs0)
+    sample = ("%d frames x %d active subapertures of %s, FP32 C restatement of centmodule.c (own radix-2 FFT, "
+              "no FFTW), %d threads, %.1f s" % (frames, nact, w.name, nthr, dt))
+    return nact * frames / dt, nthr, sample, oc
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--form", default="fft32", choices=["fft32", "bf16", "fp16"])
+    ap.add_argument("--workload", default="c3", choices=["c1", "c2", "c3", "c5"])
+    ap.add_argument("--frames", type=int, default=0, help="frames per step (default: the workload's)")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (requested --gpus %d)" % args.gpus,
+              file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from dasp_amd import workloads as W, synth
+    from dasp_amd.cmod import cent
+    from dasp_amd.util import centcmod
+
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    w = {"c1": W.C1, "c2": W.C2, "c3": W.C3, "c5": W.C5}[args.workload]
+    frames = args.frames or w.nframes
+    S = w.nsubx * w.nsubx
+    pupil = w.pupil()
+    host_phs = np.zeros((w.npup, w.npup), np.float32)
+    cents_h = np.zeros((w.nsubx, w.nsubx, 2), np.float32)
+    bimg_h = np.zeros((w.nsubx, w.nsubx, w.nimg, w.nimg), np.float32)
+    corr = corrimg = None
+    if w.correlation:
+        from dasp_amd.util.correlation import transformPSF
+        corr = transformPSF(W.reference_images(w, pupil))
+        corrimg = np.zeros_like(corr)
+    cc = centcmod.centcmod(1, w.nsubx, w.cen, w.nfft, w.clip, w.nimg, w.n, w.readnoise, 0., w.addPoisson,
+                           w.noiseFloor, float(w.sig), 0., 0, 1., 1, 0, 1, host_phs, pupil, None, cents_h, bimg_h,
+                           0.5, 0, None, w.correlation, 0., corr, corrimg, w.threshType, 0, 0, 1, w.parabolicFit,
+                           None)
+    h = cc.centstruct
+    cent.setForm(h, {"fft32": 0, "bf16": 1, "fp16": 2}[args.form])
+    nact, npup, csz = cent.info(h)
+
+    # synthetic phase frames, generated and kept in HBM
+    phs = synth.phase_screens_torch(frames, w.npup, w.D, w.r0, w.L0, w.seed, dev, wavelength_scale=500. / 640.)
+    phs[0].copy_(torch.from_numpy(w.phase(0)).to(dev))       # frame 0 = the numpy screen (error check)
+    d_cents = torch.zeros((frames, S, 2), dtype=torch.float32, device=dev)
+    noise = bool(w.addPoisson or w.readnoise)
+    zp = zr = None
+    if noise:
+        m2 = S * w.nimg * w.nimg
+        zp = torch.from_numpy(np.stack([cent.drawNormals(1, 2 * f, m2) for f in range(frames)])).to(dev)
+        zr = torch.from_numpy(np.stack([cent.drawNormals(1, 2 * f + 1, m2) for f in range(frames)])).to(dev)
+    d_corr = None
+    if w.correlation:
+        d_corr = torch.zeros((frames, S, csz, csz), dtype=torch.float32, device=dev)
+    stream = torch.cuda.current_stream()
+
+    def step():
+        cent.runDevice(h, phs.data_ptr(), npup * npup, frames, d_cents.data_ptr(), 0,
+                       d_corr.data_ptr() if d_corr is not None else 0,
+                       zp.data_ptr() if zp is not None else 0, zr.data_ptr() if zr is not None else 0,
+                       stream.cuda_stream)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    cent.setTiming(h, True)
+    cent.kernelTime(h, reset=True)
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    e0.record(stream)
+    for _ in range(args.steps):
+        step()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms = e0.elapsed_time(e1)
+    kms, klaunch = cent.kernelTime(h, reset=True)
+    cent.setTiming(h, False)
+    ms_step = ms / args.steps
+    value = nact * frames * args.steps / (ms / 1e3)
+
+    result = {
+        "metric": "subap images+centroids/s @80x80 nfft=32, plain vs MFMA-DFT; RMS err vs FP64",
+        "value": value,
+        "unit": "subap-images/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": {"fft32": "f32", "bf16": "bf16x3+f32acc", "fp16": "f16x3+f32acc"}[args.form],
+        "data": "synthetic",
+        "config": {"workload": w.name, "nsubx": w.nsubx, "n": w.n, "nfft": w.nfft, "nimg": w.nimg,
+                   "npup": w.npup, "frames_per_step": frames, "active_subaps": nact, "form": args.form,
+                   "noise": noise, "images_returned": False, "wall_s": wall},
+    }
+    if klaunch > 0:
+        kavg = kms / klaunch / 1e3
+        abytes = algorithmic_bytes(w, nact, frames, False, noise)
+        ach = abytes / kavg / 1e9
+        result["roofline"] = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                              "frac": ach / HBM_PEAK_GBS, "traffic": None,
+                              "kernel_avg_ms": kavg * 1e3, "algorithmic_bytes": abytes}
+    if not args.no_cpu:
+        # CPU leg: the FP64 oracle on frame 0 of the same workload; its slopes also give the
+        # engine's error against FP64.
+        v, nthr, sample, oc = cpu_baseline(w, w.phase(0))
+        result["cpu_baseline"] = {"value": v, "unit": "subap-images/s", "cores": nthr, "kind": "port",
+                                  "sample": sample}
+        if not noise and not w.correlation:
+            act = cc.subflag.ravel() == 1
+            d = (d_cents[0].cpu().numpy() - oc)[act]
+            result["rms_err_vs_fp64_px"] = float(np.sqrt((d ** 2).mean()))
+            result["max_err_vs_fp64_px"] = float(np.abs(d).max())
+    cc.free()
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

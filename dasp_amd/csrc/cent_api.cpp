@@ -1,0 +1,1107 @@
+// cent_api.cpp - host side of the gfx950 centroiding engine behind include/dasp_cent.h.
+//
+// Mirrors the state machine of cmod/src/centmodule.c (py_initialise :2858, py_run :3188,
+// py_runSlope :3230, py_update :2573, py_free :3163) with the per-subaperture work moved to HIP
+// kernels. Differences from the reference, all deliberate:
+//  - no host thread pool; nthreads is accepted and ignored;
+//  - noise comes from pre-drawn N(0,1) arrays (host PCG32/Box-Muller streams per readout) and
+//    the inverse transform in noise.h, instead of in-module PCG32/GSL draws;
+//  - arrays given to initialise/update are copied to HBM at that call (the phase and the input
+//    image are re-read from the borrowed buffer at every run).
+#include <hip/hip_runtime.h>
+#include <sys/time.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <ctime>
+#include <string>
+#include <vector>
+
+#include "../../include/dasp_cent.h"
+#include "cent_internal.h"
+#include "host_math.h"
+
+using namespace dasp;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char *fmt, ...) {
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+#define HIPCHK(expr)                                                                        \
+  do {                                                                                      \
+    hipError_t _e = (expr);                                                                 \
+    if (_e != hipSuccess) return fail(DASP_CENT_EHIP, "%s: %s", #expr, hipGetErrorString(_e)); \
+  } while (0)
+
+struct DevBuf {
+  void *p = nullptr;
+  size_t bytes = 0;
+  hipError_t ensure(size_t nb) {
+    if (nb <= bytes && p) return hipSuccess;
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+    if (nb == 0) return hipSuccess;
+    hipError_t e = hipMalloc(&p, nb);
+    if (e == hipSuccess) bytes = nb;
+    return e;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  template <typename T>
+  T *as() const { return reinterpret_cast<T *>(p); }
+};
+
+hipError_t upload(DevBuf &b, const void *src, size_t nb) {
+  hipError_t e = b.ensure(nb);
+  if (e != hipSuccess || nb == 0) return e;
+  return hipMemcpy(b.p, src, nb, hipMemcpyHostToDevice);
+}
+
+long numel(const dasp_cent_array &a) {
+  long n = 1;
+  for (int i = 0; i < a.ndim; ++i) n *= a.dims[i];
+  return n;
+}
+
+bool is_pow2(int v) { return v > 0 && (v & (v - 1)) == 0; }
+
+int pad_pow2(int v) {
+  int r = 8;
+  while (r < v) r <<= 1;
+  return r;
+}
+
+}  // namespace
+
+struct dasp_cent {
+  dasp_cent_params p{};
+  int nsubx = 0, nsubaps = 0, n = 0, F = 0, clip = 0, nimg = 0, ncen = 0;
+  int psf = 0, padded = 0, pbf = 1, preBinOrig = 1, spotDim = 0;
+  int corrsize = 0, corrPatternSize = 0, npup = 0;
+  int atmosPhaseType = 0, interpolate = 0;
+  int curInteg = 0, form = DASP_CENT_FORM_FFT32;
+  unsigned long seed = 1;
+  unsigned long nreadouts = 0;
+  float sig = 0.f, skybrightness = 0.f, readnoise = 0.f, readbg = 0.f, noiseFloor = 0.f;
+  float pxlPower = 1.f, corrThresh = 0.f;
+  int addPoisson = 0, calsource = 0, opticalBinning = 0, threshType = 0, imageOnly = 0;
+  int correlationCentroiding = 0, useBrightest = 0, parabolicFit = 0, gaussianFit = 0;
+  float gaussianMinVal = 0.f, gaussianReplaceVal = 0.f;
+  int nintegrations = 1, nlatency = 0;
+  int centWeightDim = 0, centWeightSize = 0;
+  int calNSteps = 0, calNCoeff = 0;
+  bool haveSigArr = false, haveSkyArr = false, haveRef = false, haveCalData = false;
+  bool haveCalCoeff = false, haveUBArr = false, haveCorr = false, haveCentWeight = false;
+  bool tablesDirty = true;
+  std::vector<int> active, inactive, subflag;
+  std::vector<float> pupfn, frac, sigArr, tilt, totsig, spot;
+  int fitMxSize = -999;
+  float fitMx[30]{};
+  // borrowed host outputs / inputs
+  float *h_phs = nullptr, *h_cents = nullptr, *h_bimg = nullptr, *h_corrimg = nullptr;
+  const float *h_input = nullptr;
+  int inputDims = 0, inputNpxlx = 0;
+  long phsElems = 0;
+  // device state
+  DevBuf d_phs, d_pup, d_tilt, d_active, d_inactive, d_scale, d_totsig, d_sigv, d_skyadd, d_skyarr;
+  DevBuf d_poisson, d_binA, d_otf, d_corrK, d_corrPat, d_centWeight, d_refCents, d_calData;
+  DevBuf d_calBounds, d_calSteps, d_calCoeff, d_ubArr, d_input, d_subapLoc, d_cents, d_bimg;
+  DevBuf d_corrimg, d_hll, d_zpois, d_zread, d_bscratch, d_cscratch;
+  std::vector<float> h_stage;
+  std::vector<float> noiseHost;
+  // kernel timing
+  bool timing = false;
+  std::vector<hipEvent_t> evs;   // start/stop pairs not yet harvested
+  double kernel_ms = 0.0;
+  long kernel_launches = 0;
+};
+
+static int harvest_events(dasp_cent *c) {
+  for (size_t i = 0; i + 1 < c->evs.size(); i += 2) {
+    HIPCHK(hipEventSynchronize(c->evs[i + 1]));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, c->evs[i], c->evs[i + 1]));
+    c->kernel_ms += ms;
+    c->kernel_launches++;
+    (void)hipEventDestroy(c->evs[i]);
+    (void)hipEventDestroy(c->evs[i + 1]);
+  }
+  c->evs.clear();
+  return 0;
+}
+
+static int timing_mark(dasp_cent *c, hipStream_t st) {
+  if (!c->timing) return 0;
+  hipEvent_t e;
+  HIPCHK(hipEventCreate(&e));
+  HIPCHK(hipEventRecord(e, st));
+  c->evs.push_back(e);
+  if (c->evs.size() >= 512 && c->evs.size() % 2 == 0) return harvest_events(c);
+  return 0;
+}
+
+// ------------------------------------------------------------------------------------------------
+// derived tables
+// ------------------------------------------------------------------------------------------------
+static int set_spot(dasp_cent *c, const dasp_cent_array &sp) {
+  // setSpotPsf (cmod/src/centmodule.c:2255-2301)
+  c->spotDim = 0;
+  c->psf = c->clip;
+  c->pbf = 1;
+  c->padded = c->clip;
+  c->spot.clear();
+  if (!sp.data) return 0;
+  if (sp.ndim == 2) {
+    c->spotDim = 2;
+    c->psf = (int)sp.dims[0];
+    if (sp.dims[1] != sp.dims[0]) return fail(DASP_CENT_EINVAL, "spotpsf must be square");
+  } else if (sp.ndim == 4) {
+    c->spotDim = 4;
+    c->psf = (int)sp.dims[2];
+    if (sp.dims[0] * sp.dims[1] != c->nsubaps || sp.dims[3] != sp.dims[2])
+      return fail(DASP_CENT_EINVAL, "spotpsf should be (nsubx,nsubx,P,P)");
+  } else {
+    return fail(DASP_CENT_EINVAL, "centmodule - spotpsf not 2d or 4d");
+  }
+  c->pbf = c->preBinOrig;
+  c->padded = c->pbf <= 1 ? c->psf : c->F;
+  if (c->pbf < 1) c->pbf = 1;
+  if (c->padded % c->pbf != 0)
+    return fail(DASP_CENT_EINVAL, "paddedsize(%d) modulo preBinningFactor(%d) should be zero", c->padded, c->pbf);
+  if (c->pbf > 1 && c->psf > c->padded)
+    return fail(DASP_CENT_EINVAL, "spotpsf dimensions should be equal to or less than paddedsize (%d)", c->padded);
+  if (c->psf < c->clip) return fail(DASP_CENT_EINVAL, "spotpsf dimensions should be at least clipsize");
+  if (!is_pow2(c->psf) || c->psf > kMaxSide)
+    return fail(DASP_CENT_EUNSUPPORTED, "spot kernel size %d: the device convolution needs a power of two <= %d", c->psf, kMaxSide);
+  if (c->padded > kMaxSide) return fail(DASP_CENT_EUNSUPPORTED, "paddedsize %d > %d", c->padded, kMaxSide);
+  const float *src = (const float *)sp.data;
+  c->spot.assign(src, src + numel(sp));
+  // OTF = r2c(flipArray(spot, mirror=1)) (prepareSpotPsf :2348-2372), stored as the full
+  // Hermitian P x P spectrum for the device's complex convolution.
+  const int P = c->psf;
+  const int nk = c->spotDim == 2 ? 1 : c->nsubaps;
+  std::vector<float2> otf((size_t)nk * P * P);
+  std::vector<float> tmp(P * P);
+  std::vector<cd> z(P * P);
+  for (int k = 0; k < nk; ++k) {
+    std::memcpy(tmp.data(), src + (long)k * P * P, sizeof(float) * P * P);
+    host_flip_array(tmp.data(), P, 1);
+    for (int q = 0; q < P * P; ++q) z[q] = cd(tmp[q], 0);
+    host_dft2(z.data(), P, P, -1);
+    for (int q = 0; q < P * P; ++q) otf[(size_t)k * P * P + q] = make_float2((float)z[q].real(), (float)z[q].imag());
+  }
+  if (upload(c->d_otf, otf.data(), otf.size() * sizeof(float2)) != hipSuccess)
+    return fail(DASP_CENT_EHIP, "upload of spot OTFs failed");
+  return 0;
+}
+
+// updateTotSigArr (cmod/src/centmodule.c:2403-2427), float arithmetic as the reference.
+static void compute_totsig(dasp_cent *c) {
+  const int n2 = c->n * c->n;
+  c->totsig.assign(c->nsubaps, 0.f);
+  for (int i = 0; i < c->nsubaps; ++i) {
+    float t = 0.f;
+    for (int j = 0; j < n2; ++j) t += c->pupfn[(long)i * n2 + j];
+    t *= (float)(c->F * c->F);
+    if (c->spotDim) {
+      const int P2 = c->psf * c->psf;
+      const float *sp = c->spot.data() + (c->spotDim == 4 ? (long)i * P2 : 0);
+      float tot = 0.f;
+      for (int j = 0; j < P2; ++j) tot += sp[j];
+      t *= tot * P2;
+    }
+    c->totsig[i] = t;
+  }
+}
+
+static int upload_tables(dasp_cent *c) {
+  if (!c->tablesDirty) return 0;
+  const int S = c->nsubaps;
+  std::vector<float> scale(S), sigv(S), skyadd(S);
+  std::vector<unsigned char> pois(S);
+  for (int i = 0; i < S; ++i) {
+    const float nph = c->frac[i];
+    sigv[i] = c->haveSigArr ? c->sigArr[i] : c->sig * nph;
+    scale[i] = c->totsig[i] > 0.f ? (c->haveSigArr ? c->sigArr[i] / c->totsig[i] : c->sig * nph / c->totsig[i]) : 1.f;
+    const float sky = c->skybrightness * nph;
+    skyadd[i] = sky > 0.f ? sky : 0.f;
+    pois[i] = (c->addPoisson == 1 && c->calsource == 0) && (c->totsig[i] > 0.f || sky > 0.f || c->haveSkyArr);
+  }
+  if (upload(c->d_scale, scale.data(), S * 4) || upload(c->d_sigv, sigv.data(), S * 4) ||
+      upload(c->d_skyadd, skyadd.data(), S * 4) || upload(c->d_poisson, pois.data(), S) ||
+      upload(c->d_totsig, c->totsig.data(), S * 4))
+    return fail(DASP_CENT_EHIP, "upload of per-subaperture tables failed");
+  // binning operator A (nimg x clip) from prepareBinImage/binImage (:565-651), float weights
+  const int Q = c->clip, M = c->nimg;
+  std::vector<float> A((size_t)M * Q, 0.f);
+  if (Q == M) {
+    for (int i = 0; i < M; ++i) A[i * Q + i] = 1.f;
+  } else {
+    std::vector<float> t(Q), nx(Q);
+    std::vector<int> idx(Q), pos(Q);
+    int cnt = 0;
+    for (int i = 0; i < Q; ++i) {
+      cnt += M;
+      if (cnt >= Q) {
+        const int ex = cnt - Q;
+        t[i] = (M - ex) / (float)M;
+        nx[i] = ex / (float)M;
+        idx[i] = 1;
+        cnt = ex;
+      } else {
+        t[i] = 1.f;
+        nx[i] = 0.f;
+        idx[i] = 0;
+      }
+    }
+    int ii = 0;
+    for (int r = 0; r < Q; ++r) {
+      pos[r] = ii;
+      ii += idx[r];
+    }
+    for (int r = 0; r < Q; ++r) {
+      if (pos[r] < M) A[pos[r] * Q + r] += t[r];
+      if (r >= 1 && pos[r - 1] + 1 < M) A[(pos[r - 1] + 1) * Q + r] += nx[r - 1];
+    }
+  }
+  if (upload(c->d_binA, A.data(), A.size() * 4)) return fail(DASP_CENT_EHIP, "upload of binning table failed");
+  // fit matrix for the effective fit size
+  int fs = c->parabolicFit + c->gaussianFit;
+  if (fs >= 0 && fs < 3) fs = 3;
+  if (fs > 0 && fs != c->fitMxSize) {
+    if (!host_fit_matrix(fs, c->fitMx)) return fail(DASP_CENT_EINVAL, "fit matrix for fitsize %d is singular", fs);
+    c->fitMxSize = fs;
+  }
+  c->tablesDirty = false;
+  return 0;
+}
+
+static int set_corr_pattern(dasp_cent *c, const dasp_cent_array &a) {
+  c->haveCorr = false;
+  if (!a.data) return 0;
+  if (a.ndim != 4 || a.dims[0] * a.dims[1] != c->nsubaps || a.dims[2] != a.dims[3])
+    return fail(DASP_CENT_EINVAL, "corrPattern should be (nsubx,nsubx,C,C)");
+  const int C = (int)a.dims[2];
+  if (C > kMaxSide) return fail(DASP_CENT_EUNSUPPORTED, "corrPattern size %d > %d", C, kMaxSide);
+  const float *src = (const float *)a.data;
+  const size_t per = (size_t)C * C;
+  if (upload(c->d_corrPat, src, per * c->nsubaps * 4)) return fail(DASP_CENT_EHIP, "corrPattern upload failed");
+  if (is_pow2(C)) {
+    std::vector<float2> K(per * c->nsubaps);
+    std::vector<cd> k(per);
+    for (int s = 0; s < c->nsubaps; ++s) {
+      host_hc_to_spectrum(src + per * s, C, C, k.data());
+      for (size_t q = 0; q < per; ++q) K[per * s + q] = make_float2((float)k[q].real(), (float)k[q].imag());
+    }
+    if (upload(c->d_corrK, K.data(), K.size() * sizeof(float2))) return fail(DASP_CENT_EHIP, "corr spectrum upload failed");
+  } else {
+    c->d_corrK.release();
+  }
+  c->haveCorr = true;
+  return 0;
+}
+
+static int set_cent_weight(dasp_cent *c, const dasp_cent_array &a) {
+  // setCentWeight (cmod/src/centmodule.c:2227-2253)
+  c->centWeightDim = 0;
+  c->haveCentWeight = false;
+  if (!a.data) return 0;
+  if (a.ndim != 2 && a.ndim != 4) return fail(DASP_CENT_EINVAL, "centmodule - centWeight not 2d or 4d");
+  c->centWeightDim = a.ndim;
+  c->centWeightSize = (int)a.dims[a.ndim - 2];
+  if (a.dims[a.ndim - 2] != c->corrsize || a.dims[a.ndim - 1] != c->corrsize)
+    return fail(DASP_CENT_EINVAL, "centweight array not correct dimensions (need to be nsubx,nsubx,corrsize,corrsize)");
+  if (upload(c->d_centWeight, a.data, numel(a) * 4)) return fail(DASP_CENT_EHIP, "centWeight upload failed");
+  c->haveCentWeight = true;
+  return 0;
+}
+
+static bool check_f32(const dasp_cent_array &a) { return a.data != nullptr; }
+
+// ------------------------------------------------------------------------------------------------
+// kernel argument assembly and launch
+// ------------------------------------------------------------------------------------------------
+static void fill_back(dasp_cent *c, BackArgs &b, int nframes) {
+  std::memset(&b, 0, sizeof(b));
+  const int M2 = c->nimg * c->nimg;
+  b.active = c->d_active.as<int>();
+  b.nactive = (int)c->active.size();
+  b.nitems = (long)nframes * b.nactive;
+  b.nsubx = c->nsubx;
+  b.nsubaps = c->nsubaps;
+  b.n = c->n;
+  b.F = c->F;
+  b.clip = c->clip;
+  b.nimg = c->nimg;
+  b.ncen = c->ncen;
+  b.psf = c->psf;
+  b.padded = c->padded;
+  b.pbf = c->spotDim ? c->pbf : 1;
+  b.corrsize = c->corrsize;
+  b.npup = c->npup;
+  b.otf = c->spotDim ? c->d_otf.as<float2>() : nullptr;
+  b.otf_per_subap = c->spotDim == 4;
+  b.binA = c->d_binA.as<float>();
+  b.scale = c->d_scale.as<float>();
+  b.totsig = c->d_totsig.as<float>();
+  b.sigv = c->d_sigv.as<float>();
+  b.pup = c->d_pup.as<float>();
+  b.skyadd = c->d_skyadd.as<float>();
+  b.skyarr = c->haveSkyArr ? c->d_skyarr.as<float>() : nullptr;
+  b.poisson_on = c->d_poisson.as<unsigned char>();
+  b.noise_stride = (long)c->nsubaps * M2;
+  b.readnoise = c->calsource == 0 ? c->readnoise : 0.f;
+  b.readbg = c->readbg;
+  b.floor = c->noiseFloor;
+  b.threshType = c->threshType;
+  b.opticalBinning = c->opticalBinning;
+  b.calsource = c->calsource;
+  b.useBrightest = c->useBrightest;
+  b.useBrightestArr = c->haveUBArr ? c->d_ubArr.as<int>() : nullptr;
+  b.pxlPower = c->pxlPower;
+  b.imageOnly = c->imageOnly;
+  b.corrMode = c->haveCorr ? c->correlationCentroiding : 0;
+  if (c->correlationCentroiding == 5 && !c->haveCorr) b.corrMode = 0;
+  b.corrThresh = c->corrThresh;
+  b.corrK = (c->haveCorr && c->correlationCentroiding == 1) ? c->d_corrK.as<float2>() : nullptr;
+  b.corrPattern = c->haveCorr ? c->d_corrPat.as<float>() : nullptr;
+  b.centWeight = c->haveCentWeight ? c->d_centWeight.as<float>() : nullptr;
+  b.centWeightDim = c->haveCentWeight ? c->centWeightDim : 0;
+  {
+    int fs = c->parabolicFit + c->gaussianFit;
+    b.fitsize = fs < 0 ? -1 : (fs < 3 ? 3 : fs);
+  }
+  b.doFit = (c->parabolicFit != 0 || c->gaussianFit > 0);
+  b.gaussianFit = c->gaussianFit;
+  b.gaussianMinVal = c->gaussianMinVal;
+  b.gaussianReplaceVal = c->gaussianReplaceVal;
+  std::memcpy(b.fitMx, c->fitMx, sizeof(b.fitMx));
+  b.calData = c->haveCalData ? c->d_calData.as<float>() : nullptr;
+  b.calBounds = c->d_calBounds.as<int>();
+  b.calSteps = c->d_calSteps.as<float>();
+  b.calNSteps = c->calNSteps;
+  b.calCoeff = c->haveCalCoeff ? c->d_calCoeff.as<float>() : nullptr;
+  b.calNCoeff = c->calNCoeff;
+  b.refCents = c->haveRef ? c->d_refCents.as<float>() : nullptr;
+  b.cents_stride = (long)c->nsubaps * 2;
+  b.bimg_stride = (long)c->nsubaps * M2;
+  b.corr_stride = (long)c->nsubaps * c->corrsize * c->corrsize;
+}
+
+static bool need_pois(dasp_cent *c) { return c->addPoisson == 1 && c->calsource == 0; }
+static bool need_read(dasp_cent *c) { return c->readnoise != 0.f && c->calsource == 0; }
+
+// The fused kernel covers: no spot kernel, clip == fftsize, integer binning of a shape it was
+// instantiated for, phase only, plain or weighted CoG without calibration or pixel selection.
+static bool fused_options(dasp_cent *c) {
+  return c->spotDim == 0 && c->clip == c->F && c->atmosPhaseType == 0 && c->opticalBinning == 0 &&
+         c->imageOnly == 0 && c->correlationCentroiding == 0 && c->parabolicFit == 0 && c->gaussianFit == 0 &&
+         !c->haveCalData && !c->haveCalCoeff && (c->calsource != 0 || (c->useBrightest == 0 && !c->haveUBArr));
+}
+
+// The DFT-GEMM kernel covers the fused options for n = 16, fftsize = 32, nimg = 16.
+static bool use_dft(dasp_cent *c) {
+  return (c->form == DASP_CENT_FORM_DFT_FP16 || c->form == DASP_CENT_FORM_DFT_BF16) && fused_options(c) &&
+         dft_supported(c->n, c->F, c->nimg);
+}
+
+static bool use_fused(dasp_cent *c) {
+  return c->form != DASP_CENT_FORM_FFT32_GENERAL && fused_options(c) &&
+         (use_dft(c) || fused_supported(pad_pow2(c->n), c->F, c->nimg));
+}
+
+// Phase -> centroids for nframes frames already resident on the device.
+static int launch_phase(dasp_cent *c, const float *d_phs, long phs_stride, int nframes, float *d_cents,
+                        float *d_bimg, float *d_corrimg, const float *d_zpois, const float *d_zread,
+                        hipStream_t st) {
+  int rc = upload_tables(c);
+  if (rc) return rc;
+  const long nitems = (long)nframes * (long)c->active.size();
+  const int np = pad_pow2(c->n);
+  const float *amp = c->atmosPhaseType == 1 ? d_phs + (long)c->npup * c->npup : nullptr;
+  if (nitems > 0 && use_fused(c)) {
+    FusedArgs fa{};
+    fa.phs = d_phs;
+    fa.phs_stride = phs_stride;
+    fa.pup = c->d_pup.as<float>();
+    fa.tilt = c->d_tilt.as<float>();
+    fa.active = c->d_active.as<int>();
+    fa.nactive = (int)c->active.size();
+    fa.nitems = nitems;
+    fa.nsubx = c->nsubx;
+    fa.n = c->n;
+    fa.npup = c->npup;
+    fa.nimg = c->nimg;
+    fa.ncen = c->ncen;
+    fa.scale = c->d_scale.as<float>();
+    fa.skyadd = c->d_skyadd.as<float>();
+    fa.skyarr = c->haveSkyArr ? c->d_skyarr.as<float>() : nullptr;
+    fa.poisson_on = c->d_poisson.as<unsigned char>();
+    fa.zpois = d_zpois;
+    fa.zread = d_zread;
+    fa.noise_stride = (long)c->nsubaps * c->nimg * c->nimg;
+    fa.readnoise = c->calsource == 0 ? c->readnoise : 0.f;
+    fa.readbg = c->readbg;
+    fa.floor = c->noiseFloor;
+    fa.threshType = c->threshType;
+    fa.calsource = c->calsource;
+    fa.pxlPower = c->pxlPower;
+    fa.centWeight = c->haveCentWeight ? c->d_centWeight.as<float>() : nullptr;
+    fa.centWeightDim = c->haveCentWeight ? c->centWeightDim : 0;
+    fa.refCents = c->haveRef ? c->d_refCents.as<float>() : nullptr;
+    fa.cents = d_cents;
+    fa.cents_stride = (long)c->nsubaps * 2;
+    fa.bimg = d_bimg;
+    fa.bimg_stride = (long)c->nsubaps * c->nimg * c->nimg;
+    int trc = timing_mark(c, st);
+    if (trc) return trc;
+    if (use_dft(c)) HIPCHK(launch_dft(fa, c->form == DASP_CENT_FORM_DFT_BF16, st));
+    else HIPCHK(launch_fused(fa, np, c->F, st));
+    trc = timing_mark(c, st);
+    if (trc) return trc;
+  } else if (nitems > 0) {
+    const long hll_elems = nitems * (long)c->padded * c->padded;
+    HIPCHK(c->d_hll.ensure(hll_elems * sizeof(float)));
+    if (c->padded > c->F) HIPCHK(launch_fill(c->d_hll.as<float>(), hll_elems, 0.f, st));
+    FormArgs fa{};
+    fa.phs = d_phs;
+    fa.amp = amp;
+    fa.phs_stride = phs_stride;
+    fa.pup = c->d_pup.as<float>();
+    fa.tilt = c->d_tilt.as<float>();
+    fa.active = c->d_active.as<int>();
+    fa.nactive = (int)c->active.size();
+    fa.nitems = nitems;
+    fa.nsubx = c->nsubx;
+    fa.n = c->n;
+    fa.npup = c->npup;
+    fa.padded = c->padded;
+    fa.hll = c->d_hll.as<float>();
+    int trc = timing_mark(c, st);
+    if (trc) return trc;
+    HIPCHK(launch_form(fa, np, c->F, st));
+    trc = timing_mark(c, st);
+    if (trc) return trc;
+    BackArgs b;
+    fill_back(c, b, nframes);
+    b.hll = c->d_hll.as<float>();
+    b.amp = amp;
+    b.phs_stride = phs_stride;
+    b.zpois = d_zpois;
+    b.zread = d_zread;
+    b.cents = d_cents;
+    b.bimg = d_bimg;
+    b.corrimg = d_corrimg;
+    HIPCHK(launch_backend(b, st));
+  }
+  if (!c->inactive.empty() && (c->imageOnly == 0 ? d_cents != nullptr : d_bimg != nullptr)) {
+    InactiveArgs ia{};
+    ia.inactive = c->d_inactive.as<int>();
+    ia.ninactive = (int)c->inactive.size();
+    ia.nframes = nframes;
+    ia.nsubaps = c->nsubaps;
+    ia.nimg = c->nimg;
+    ia.imageOnly = c->imageOnly;
+    ia.readnoise = c->calsource == 0 ? c->readnoise : 0.f;
+    ia.readbg = c->readbg;
+    ia.floor = c->noiseFloor;
+    ia.threshType = c->threshType;
+    ia.zread = d_zread;
+    ia.noise_stride = (long)c->nsubaps * c->nimg * c->nimg;
+    ia.cents = d_cents;
+    ia.cents_stride = (long)c->nsubaps * 2;
+    ia.bimg = d_bimg;
+    ia.bimg_stride = (long)c->nsubaps * c->nimg * c->nimg;
+    HIPCHK(launch_inactive(ia, st));
+  }
+  return 0;
+}
+
+// Pre-draw the readout's noise on the host and copy it to the device.
+static int draw_noise(dasp_cent *c, bool pois, bool rd) {
+  const long per = (long)c->nsubaps * c->nimg * c->nimg;
+  const unsigned long k = c->nreadouts;
+  c->noiseHost.resize(per);
+  if (pois) {
+    host_draw_normals(c->seed, 2 * k, per, c->noiseHost.data());
+    if (upload(c->d_zpois, c->noiseHost.data(), per * 4)) return fail(DASP_CENT_EHIP, "noise upload failed");
+  }
+  if (rd) {
+    host_draw_normals(c->seed, 2 * k + 1, per, c->noiseHost.data());
+    if (upload(c->d_zread, c->noiseHost.data(), per * 4)) return fail(DASP_CENT_EHIP, "noise upload failed");
+  }
+  return 0;
+}
+
+static double now_s() {
+  struct timeval t;
+  gettimeofday(&t, nullptr);
+  return t.tv_sec + 1e-6 * t.tv_usec;
+}
+
+// Copy device outputs into the borrowed host arrays with the reference's write pattern.
+static int copy_back(dasp_cent *c) {
+  const int M2 = c->nimg * c->nimg;
+  if (c->imageOnly == 0 && c->h_cents)
+    HIPCHK(hipMemcpy(c->h_cents, c->d_cents.p, (size_t)c->nsubaps * 2 * 4, hipMemcpyDeviceToHost));
+  if (c->h_bimg) {
+    const size_t nb = (size_t)c->nsubaps * M2 * 4;
+    if (c->imageOnly) {
+      HIPCHK(hipMemcpy(c->h_bimg, c->d_bimg.p, nb, hipMemcpyDeviceToHost));
+    } else {
+      c->h_stage.resize((size_t)c->nsubaps * M2);
+      HIPCHK(hipMemcpy(c->h_stage.data(), c->d_bimg.p, nb, hipMemcpyDeviceToHost));
+      for (int s : c->active) std::memcpy(c->h_bimg + (size_t)s * M2, c->h_stage.data() + (size_t)s * M2, M2 * 4);
+    }
+  }
+  if (c->h_corrimg && c->haveCorr && c->correlationCentroiding && c->imageOnly == 0) {
+    const int C2 = c->corrsize * c->corrsize;
+    c->h_stage.resize((size_t)c->nsubaps * C2);
+    HIPCHK(hipMemcpy(c->h_stage.data(), c->d_corrimg.p, (size_t)c->nsubaps * C2 * 4, hipMemcpyDeviceToHost));
+    for (int s : c->active) std::memcpy(c->h_corrimg + (size_t)s * C2, c->h_stage.data() + (size_t)s * C2, C2 * 4);
+  }
+  return 0;
+}
+
+// ------------------------------------------------------------------------------------------------
+// C ABI
+// ------------------------------------------------------------------------------------------------
+extern "C" {
+
+const char *dasp_cent_last_error(void) { return g_err.c_str(); }
+int dasp_cent_abi_version(void) { return DASP_CENT_ABI_VERSION; }
+
+int dasp_cent_draw_normals(unsigned long seed, unsigned long stream, long n, float *out) {
+  if (n < 0 || (n > 0 && !out)) return fail(DASP_CENT_EINVAL, "bad arguments");
+  host_draw_normals(seed, stream, n, out);
+  return 0;
+}
+
+int dasp_cent_hc_to_spectrum(const float *pattern_hc, int C, float *spectrum) {
+  if (!pattern_hc || !spectrum || C <= 0) return fail(DASP_CENT_EINVAL, "bad arguments");
+  std::vector<cd> k((size_t)C * C);
+  host_hc_to_spectrum(pattern_hc, C, C, k.data());
+  for (size_t q = 0; q < k.size(); ++q) {
+    spectrum[2 * q] = (float)k[q].real();
+    spectrum[2 * q + 1] = (float)k[q].imag();
+  }
+  return 0;
+}
+
+int dasp_cent_initialise(const dasp_cent_params *pp, dasp_cent_t **out) {
+  if (!pp || !out) return fail(DASP_CENT_EINVAL, "null argument");
+  *out = nullptr;
+  const dasp_cent_params &p = *pp;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(DASP_CENT_ENODEV, "no HIP device");
+  {
+    int dev = 0;
+    hipDeviceProp_t prop;
+    if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess &&
+        std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+      return fail(DASP_CENT_ENODEV, "device %s is not gfx950", prop.gcnArchName);
+  }
+  dasp_cent *c = new dasp_cent();
+  auto bail = [&](int rc) {
+    dasp_cent_free(c);
+    return rc;
+  };
+  c->p = p;
+  c->nsubaps = p.nsubaps;
+  c->nsubx = (int)std::sqrt((float)p.nsubaps);
+  if (c->nsubx * c->nsubx != p.nsubaps) return bail(fail(DASP_CENT_EINVAL, "nsubaps must be a square"));
+  c->ncen = p.ncen;
+  c->F = p.fftsize;
+  c->clip = p.clipsize;
+  c->nimg = p.nimg;
+  c->n = p.phasesize;
+  c->npup = c->nsubx * c->n;
+  c->preBinOrig = p.preBinningFactor;
+  c->parabolicFit = p.parabolicFit;
+  c->gaussianFit = p.gaussianFit;
+  c->gaussianMinVal = p.gaussianMinVal;
+  c->gaussianReplaceVal = p.gaussianReplaceVal;
+  c->readnoise = p.readnoise;
+  c->readbg = p.readbg;
+  c->threshType = p.threshType;
+  c->addPoisson = p.addPoisson;
+  c->noiseFloor = p.noiseFloor;
+  c->calsource = p.calsource;
+  c->pxlPower = p.pxlPower;
+  c->nintegrations = p.nintegrations;
+  c->nlatency = p.nlatency;
+  c->seed = p.seed == 0 ? (unsigned long)time(nullptr) : p.seed;
+  c->opticalBinning = p.opticalBinning;
+  c->imageOnly = p.imageOnly;
+  c->corrThresh = p.corrThresh;
+  c->correlationCentroiding = p.correlationCentroiding;
+  if (c->F <= 0 || c->n <= 0 || c->clip <= 0 || c->nimg <= 0 || c->ncen <= 0)
+    return bail(fail(DASP_CENT_EINVAL, "sizes must be positive"));
+  if (!is_pow2(c->F) || c->F < 8 || c->F > 64 || pad_pow2(c->n) > c->F || c->n > 32)
+    return bail(fail(DASP_CENT_EUNSUPPORTED, "fftsize %d / phasesize %d: the device path handles power-of-two fftsize 8..64 with phasesize <= min(32, fftsize)", c->F, c->n));
+  if (c->clip > c->F && !p.spotpsf.data)
+    return bail(fail(DASP_CENT_EINVAL, "clipsize %d > fftsize %d", c->clip, c->F));
+  if (c->nimg > c->clip || c->nimg > kMaxSide) return bail(fail(DASP_CENT_EUNSUPPORTED, "nimg %d must be <= clipsize and <= %d", c->nimg, kMaxSide));
+  if (c->ncen > c->nimg) return bail(fail(DASP_CENT_EINVAL, "ncen %d > nimg %d", c->ncen, c->nimg));
+  int rc = set_spot(c, p.spotpsf);
+  if (rc) return bail(rc);
+  // outputs and per-subaperture inputs (checkFloatContigArr / checkIntContigArr at :2132-2149)
+  if (!check_f32(p.cents) || numel(p.cents) != (long)c->nsubaps * 2) return bail(fail(DASP_CENT_EINVAL, "cents must be float32 (nsubx,nsubx,2)"));
+  if (!p.pupfn.data || numel(p.pupfn) != (long)c->nsubaps * c->n * c->n) return bail(fail(DASP_CENT_EINVAL, "pupfn must be float32 (nsubx,nsubx,n,n)"));
+  if (!p.subflag.data || numel(p.subflag) != c->nsubaps) return bail(fail(DASP_CENT_EINVAL, "subflag must be int32 (nsubx,nsubx)"));
+  if (!p.bimg.data || numel(p.bimg) != (long)c->nsubaps * c->nimg * c->nimg) return bail(fail(DASP_CENT_EINVAL, "bimg must be float32 (nsubx,nsubx,nimg,nimg)"));
+  if (!p.fracSubArea.data || numel(p.fracSubArea) != c->nsubaps) return bail(fail(DASP_CENT_EINVAL, "fracSubArea must be float32 (nsubx,nsubx)"));
+  c->h_cents = (float *)p.cents.data;
+  c->h_bimg = (float *)p.bimg.data;
+  c->pupfn.assign((const float *)p.pupfn.data, (const float *)p.pupfn.data + numel(p.pupfn));
+  c->frac.assign((const float *)p.fracSubArea.data, (const float *)p.fracSubArea.data + c->nsubaps);
+  c->subflag.assign((const int *)p.subflag.data, (const int *)p.subflag.data + c->nsubaps);
+  // phase
+  if (p.phs.data) {
+    if (p.phs.ndim == 2) c->atmosPhaseType = 0;
+    else if (p.phs.ndim == 3) c->atmosPhaseType = 1;
+    else return bail(fail(DASP_CENT_EINVAL, "centmodule - phase should be 2D or 3D (for phase+amp)"));
+    const int o = c->atmosPhaseType;
+    if (p.phs.dims[o] != c->npup || p.phs.dims[o + 1] != c->npup)
+      return bail(fail(DASP_CENT_EUNSUPPORTED, "phase grid %ldx%ld differs from nsubx*phasesize=%d: bicubic interpolation (interpolatePhase) is not implemented on the device", p.phs.dims[o], p.phs.dims[o + 1], c->npup));
+    if (o == 1 && p.phs.dims[0] != 2) return bail(fail(DASP_CENT_EINVAL, "phase+amplitude should be (2,npup,npup)"));
+    c->h_phs = (float *)p.phs.data;
+    c->phsElems = numel(p.phs);
+  }
+  // sig
+  c->sig = p.sig;
+  if (p.sigArr.data) {
+    if (p.sigArr.ndim != 1 || p.sigArr.dims[0] != c->nsubaps) return bail(fail(DASP_CENT_EINVAL, "centmodule - sig arr dimensions must be 1D (nsubx^2)"));
+    c->sigArr.assign((const float *)p.sigArr.data, (const float *)p.sigArr.data + c->nsubaps);
+    c->haveSigArr = true;
+  }
+  // sky
+  c->skybrightness = p.skybrightness;
+  if (p.skybrightnessArr.data) {
+    const auto &a = p.skybrightnessArr;
+    if (a.ndim != 3 || a.dims[0] != c->nsubaps || a.dims[1] != c->nimg || a.dims[2] != c->nimg)
+      return bail(fail(DASP_CENT_EINVAL, "centmodule - skybrightness should be 3D array shape nsubaps,nimg,nimg"));
+    c->skybrightness = 0.f;
+    if (upload(c->d_skyarr, a.data, numel(a) * 4)) return bail(fail(DASP_CENT_EHIP, "upload failed"));
+    c->haveSkyArr = true;
+  }
+  // correlation
+  if (p.corrPattern.data) {
+    c->corrPatternSize = (int)p.corrPattern.dims[2];
+    rc = set_corr_pattern(c, p.corrPattern);
+    if (rc) return bail(rc);
+  }
+  if (p.corrimg.data) c->h_corrimg = (float *)p.corrimg.data;
+  c->corrsize = c->correlationCentroiding ? c->corrPatternSize : c->nimg;
+  if (c->correlationCentroiding == 1 && c->haveCorr && !is_pow2(c->corrsize))
+    return bail(fail(DASP_CENT_EUNSUPPORTED, "FFT correlation on the device needs a power-of-two pattern size (got %d)", c->corrsize));
+  // useBrightest
+  c->useBrightest = p.useBrightest;
+  if (p.useBrightestArr.data) {
+    if (p.useBrightestArr.ndim != 1 || p.useBrightestArr.dims[0] != c->nsubaps)
+      return bail(fail(DASP_CENT_EINVAL, "centmodule - useBrightest array should be 1D of shape nsubaps (nsubx^2)"));
+    c->useBrightest = 0;
+    if (upload(c->d_ubArr, p.useBrightestArr.data, c->nsubaps * 4)) return bail(fail(DASP_CENT_EHIP, "upload failed"));
+    c->haveUBArr = true;
+  }
+  rc = set_cent_weight(c, p.centWeight);
+  if (rc) return bail(rc);
+  // image input (:3118-3157)
+  if (p.inputImage.data) {
+    const auto &a = p.inputImage;
+    if (a.ndim == 2) {
+      if (a.dims[0] * a.dims[1] != (long)c->nimg * c->nimg * c->nsubaps)
+        return bail(fail(DASP_CENT_EINVAL, "centmodule - inputImage should be %d x %d", c->nimg * c->nsubx, c->nimg * c->nsubx));
+      c->inputDims = 2;
+      c->inputNpxlx = (int)a.dims[1];
+    } else if (a.ndim == 4) {
+      if (a.dims[0] * a.dims[1] != c->nsubaps || a.dims[2] != c->nimg || a.dims[3] != c->nimg)
+        return bail(fail(DASP_CENT_EINVAL, "centmodule - inputImage should be (nsubx,nsubx,nimg,nimg)"));
+      c->inputDims = 4;
+      c->inputNpxlx = (int)a.dims[3];
+    } else {
+      return bail(fail(DASP_CENT_EINVAL, "inputImage must be 2-D or 4-D"));
+    }
+    c->h_input = (const float *)a.data;
+  }
+  if (p.subapLocation.data) {
+    if (numel(p.subapLocation) != (long)c->nsubaps * 4)
+      return bail(fail(DASP_CENT_EINVAL, "centmodule - subapLocation array should be of size %d", c->nsubaps * 4));
+    // the device copies each window into an nimg x nimg image: keep every window inside both
+    const int *loc = (const int *)p.subapLocation.data;
+    const long rows = c->inputDims == 2 ? p.inputImage.dims[0] : 0, cols = c->inputNpxlx;
+    for (int s = 0; s < c->nsubaps && c->inputDims == 2; ++s) {
+      const int r0 = loc[4 * s], r1 = loc[4 * s + 1], c0 = loc[4 * s + 2], c1 = loc[4 * s + 3];
+      if (r0 < 0 || c0 < 0 || r1 < r0 || c1 < c0 || r1 > rows || c1 > cols || r1 - r0 > c->nimg || c1 - c0 > c->nimg)
+        return bail(fail(DASP_CENT_EINVAL, "subapLocation of subaperture %d (%d,%d,%d,%d) is outside the image or larger than nimg", s, r0, r1, c0, c1));
+    }
+    if (upload(c->d_subapLoc, p.subapLocation.data, (size_t)c->nsubaps * 16)) return bail(fail(DASP_CENT_EHIP, "upload failed"));
+  }
+  // tilt function (:3074-3081): tmp is a float in the reference
+  c->tilt.resize((size_t)c->n * c->n);
+  const float tmp = (float)(M_PI / c->F * (c->F + 1.));
+  for (int i = 0; i < c->n; ++i)
+    for (int j = 0; j < c->n; ++j) c->tilt[i * c->n + j] = tmp * (float)(i + j + 1 - c->n);
+  for (int s = 0; s < c->nsubaps; ++s) (c->subflag[s] == 1 ? c->active : c->inactive).push_back(s);
+  compute_totsig(c);
+  const int M2 = c->nimg * c->nimg;
+  if (upload(c->d_pup, c->pupfn.data(), c->pupfn.size() * 4) || upload(c->d_tilt, c->tilt.data(), c->tilt.size() * 4) ||
+      upload(c->d_active, c->active.data(), c->active.size() * 4) ||
+      upload(c->d_inactive, c->inactive.data(), c->inactive.size() * 4) ||
+      c->d_cents.ensure((size_t)c->nsubaps * 2 * 4) || upload(c->d_bimg, c->h_bimg, (size_t)c->nsubaps * M2 * 4))
+    return bail(fail(DASP_CENT_EHIP, "device allocation/upload failed"));
+  if (c->corrPatternSize > 0 && c->d_corrimg.ensure((size_t)c->nsubaps * c->corrPatternSize * c->corrPatternSize * 4))
+    return bail(fail(DASP_CENT_EHIP, "device allocation failed"));
+  if (c->h_phs && c->d_phs.ensure(c->phsElems * 4)) return bail(fail(DASP_CENT_EHIP, "device allocation failed"));
+  rc = upload_tables(c);
+  if (rc) return bail(rc);
+  *out = c;
+  return 0;
+}
+
+int dasp_cent_free(dasp_cent_t *c) {
+  if (!c) return 0;
+  for (hipEvent_t e : c->evs) (void)hipEventDestroy(e);
+  c->evs.clear();
+  DevBuf *bufs[] = {&c->d_phs, &c->d_pup, &c->d_tilt, &c->d_active, &c->d_inactive, &c->d_scale, &c->d_totsig,
+                    &c->d_sigv, &c->d_skyadd, &c->d_skyarr, &c->d_poisson, &c->d_binA, &c->d_otf, &c->d_corrK,
+                    &c->d_corrPat, &c->d_centWeight, &c->d_refCents, &c->d_calData, &c->d_calBounds,
+                    &c->d_calSteps, &c->d_calCoeff, &c->d_ubArr, &c->d_input, &c->d_subapLoc, &c->d_cents,
+                    &c->d_bimg, &c->d_corrimg, &c->d_hll, &c->d_zpois, &c->d_zread, &c->d_bscratch, &c->d_cscratch};
+  for (DevBuf *b : bufs) b->release();
+  delete c;
+  return 0;
+}
+
+int dasp_cent_set_form(dasp_cent_t *c, int form) {
+  if (!c) return fail(DASP_CENT_EINVAL, "null handle");
+  if (form < DASP_CENT_FORM_FFT32 || form > DASP_CENT_FORM_FFT32_GENERAL) return fail(DASP_CENT_EINVAL, "unknown form %d", form);
+  if ((form == DASP_CENT_FORM_DFT_FP16 || form == DASP_CENT_FORM_DFT_BF16) &&
+      !(fused_options(c) && dft_supported(c->n, c->F, c->nimg)))
+    return fail(DASP_CENT_EUNSUPPORTED, "the DFT-GEMM form covers n=16, fftsize=clipsize=32, nimg=16 with a (weighted) CoG, "
+                                        "no spot kernel, phase only");
+  c->form = form;
+  return 0;
+}
+
+int dasp_cent_info(dasp_cent_t *c, int *nactive, int *npup, int *corrsize) {
+  if (!c) return fail(DASP_CENT_EINVAL, "null handle");
+  if (nactive) *nactive = (int)c->active.size();
+  if (npup) *npup = c->npup;
+  if (corrsize) *corrsize = c->corrsize;
+  return 0;
+}
+
+int dasp_cent_set_timing(dasp_cent_t *c, int enable) {
+  if (!c) return fail(DASP_CENT_EINVAL, "null handle");
+  c->timing = enable != 0;
+  return 0;
+}
+
+int dasp_cent_kernel_time(dasp_cent_t *c, int reset, double *ms, long *launches) {
+  if (!c) return fail(DASP_CENT_EINVAL, "null handle");
+  int rc = harvest_events(c);
+  if (rc) return rc;
+  if (ms) *ms = c->kernel_ms;
+  if (launches) *launches = c->kernel_launches;
+  if (reset) {
+    c->kernel_ms = 0.0;
+    c->kernel_launches = 0;
+  }
+  return 0;
+}
+
+int dasp_cent_run(dasp_cent_t *c, float *seconds) {
+  if (!c) return fail(DASP_CENT_EINVAL, "null handle");
+  const double t0 = now_s();
+  // integration cycle (:1774-1788): only the readout step produces images and slopes
+  int additive;
+  if (c->curInteg == 0) additive = (c->nintegrations + c->nlatency == 1) ? -1 : 0;
+  else if (c->curInteg + 1 < c->nintegrations) additive = 1;
+  else if (c->curInteg + 1 < c->nintegrations + c->nlatency) additive = 3;
+  else additive = 2;
+  if (additive == 2 || additive == -1) {
+    if (!c->h_phs) return fail(DASP_CENT_EINVAL, "no phase array (image-input module: use runSlope)");
+    HIPCHK(hipMemcpy(c->d_phs.p, c->h_phs, c->phsElems * 4, hipMemcpyHostToDevice));
+    const bool pois = need_pois(c), rd = need_read(c);
+    int rc = draw_noise(c, pois, rd);
+    if (rc) return rc;
+    rc = launch_phase(c, c->d_phs.as<float>(), 0, 1, c->d_cents.as<float>(), c->d_bimg.as<float>(),
+                      c->d_corrimg.as<float>(), pois ? c->d_zpois.as<float>() : nullptr,
+                      rd ? c->d_zread.as<float>() : nullptr, nullptr);
+    if (rc) return rc;
+    HIPCHK(hipDeviceSynchronize());
+    rc = copy_back(c);
+    if (rc) return rc;
+    c->nreadouts++;
+  }
+  c->curInteg++;
+  if (c->curInteg >= c->nintegrations + c->nlatency) c->curInteg = 0;
+  if (seconds) *seconds = (float)(now_s() - t0);
+  return 0;
+}
+
+int dasp_cent_run_slope(dasp_cent_t *c, float *seconds) {
+  if (!c) return fail(DASP_CENT_EINVAL, "null handle");
+  if (!c->h_input) return fail(DASP_CENT_EINVAL, "centmodule - inputImage is NULL");
+  const double t0 = now_s();
+  int rc = upload_tables(c);
+  if (rc) return rc;
+  const int M2 = c->nimg * c->nimg;
+  const long nin = (long)c->nsubaps * M2;
+  if (upload(c->d_input, c->h_input, nin * 4)) return fail(DASP_CENT_EHIP, "input image upload failed");
+  // pixels outside a subapLocation window keep their previous value (:2016-2021)
+  HIPCHK(hipMemcpy(c->d_bimg.p, c->h_bimg, nin * 4, hipMemcpyHostToDevice));
+  const bool rd = need_read(c) && c->imageOnly && !c->inactive.empty();
+  rc = draw_noise(c, false, rd);
+  if (rc) return rc;
+  BackArgs b;
+  fill_back(c, b, 1);
+  b.hll = nullptr;
+  b.inputImage = c->d_input.as<float>();
+  b.inputImageDims = c->inputDims;
+  b.inputImagenpxlx = c->inputNpxlx;
+  b.subapLocation = c->p.subapLocation.data ? c->d_subapLoc.as<int>() : nullptr;
+  b.cents = c->d_cents.as<float>();
+  b.bimg = c->d_bimg.as<float>();
+  b.corrimg = c->d_corrimg.as<float>();
+  HIPCHK(launch_backend(b, nullptr));
+  if (!c->inactive.empty()) {
+    InactiveArgs ia{};
+    ia.inactive = c->d_inactive.as<int>();
+    ia.ninactive = (int)c->inactive.size();
+    ia.nframes = 1;
+    ia.nsubaps = c->nsubaps;
+    ia.nimg = c->nimg;
+    ia.imageOnly = c->imageOnly;
+    ia.readnoise = c->calsource == 0 ? c->readnoise : 0.f;
+    ia.readbg = c->readbg;
+    ia.floor = c->noiseFloor;
+    ia.threshType = c->threshType;
+    ia.zread = rd ? c->d_zread.as<float>() : nullptr;
+    ia.noise_stride = nin;
+    ia.cents = c->d_cents.as<float>();
+    ia.cents_stride = (long)c->nsubaps * 2;
+    ia.bimg = c->d_bimg.as<float>();
+    ia.bimg_stride = nin;
+    HIPCHK(launch_inactive(ia, nullptr));
+  }
+  HIPCHK(hipDeviceSynchronize());
+  rc = copy_back(c);
+  if (rc) return rc;
+  if (rd) c->nreadouts++;
+  if (seconds) *seconds = (float)(now_s() - t0);
+  return 0;
+}
+
+int dasp_cent_run_device(dasp_cent_t *c, const float *d_phs, long phs_stride, int nframes, float *d_cents,
+                         float *d_bimg, float *d_corrimg, const float *d_zpois, const float *d_zread,
+                         void *stream) {
+  if (!c) return fail(DASP_CENT_EINVAL, "null handle");
+  if (!d_phs || nframes < 1 || !d_cents) return fail(DASP_CENT_EINVAL, "d_phs, d_cents and nframes >= 1 are required");
+  if (c->nintegrations + c->nlatency != 1) return fail(DASP_CENT_EINVAL, "run_device treats every frame as a readout (nintegrations+nlatency must be 1)");
+  if (phs_stride < (long)c->npup * c->npup * (c->atmosPhaseType + 1)) return fail(DASP_CENT_EINVAL, "phs_stride too small");
+  if (need_pois(c) && !d_zpois) return fail(DASP_CENT_EINVAL, "Poisson noise is on: d_zpois is required");
+  if (need_read(c) && !d_zread) return fail(DASP_CENT_EINVAL, "read noise is on: d_zread is required");
+  float *bimg = d_bimg;
+  if (!bimg && !use_fused(c)) {   // images not wanted: the general path still materialises them
+    HIPCHK(c->d_bscratch.ensure((size_t)nframes * c->nsubaps * c->nimg * c->nimg * 4));
+    bimg = c->d_bscratch.as<float>();
+  }
+  float *corr = d_corrimg;
+  if (!corr && c->correlationCentroiding && c->haveCorr) {
+    HIPCHK(c->d_cscratch.ensure((size_t)nframes * c->nsubaps * c->corrsize * c->corrsize * 4));
+    corr = c->d_cscratch.as<float>();
+  }
+  return launch_phase(c, d_phs, phs_stride, nframes, d_cents, bimg, corr, d_zpois, d_zread, (hipStream_t)stream);
+}
+
+int dasp_cent_update_int(dasp_cent_t *c, int code, long v) {
+  if (!c) return fail(DASP_CENT_EINVAL, "null handle");
+  switch (code) {
+    case DASP_CENT_CALSOURCE: c->calsource = (int)v; break;
+    case DASP_CENT_ADDPOISSON: c->addPoisson = (int)v; break;
+    case DASP_CENT_NCEN:
+      if (v < 1 || v > c->nimg) return fail(DASP_CENT_EINVAL, "ncen %ld out of range", v);
+      c->ncen = (int)v;
+      break;
+    case DASP_CENT_NTHREADS: break;   // no host pool
+    case DASP_CENT_OPTICALBINNING: c->opticalBinning = (int)v; break;
+    case DASP_CENT_CORRELATIONCENTROIDING:
+      c->correlationCentroiding = (int)v;
+      c->corrsize = v ? c->corrPatternSize : c->nimg;
+      if (c->haveCentWeight && c->centWeightSize != c->corrsize) {
+        std::fprintf(stderr, "WARNING: centWeightSize INCORRECT: Ignoring centWeight until updated.\n");
+        c->haveCentWeight = false;
+      }
+      break;
+    case DASP_CENT_USEBRIGHTEST:
+      c->useBrightest = (int)v;
+      c->haveUBArr = false;
+      break;
+    case DASP_CENT_INTEGSTEPS: c->nintegrations = (int)v; break;
+    case DASP_CENT_READNOISE: c->readnoise = (float)v; break;
+    case DASP_CENT_READBG: c->readbg = (float)v; break;
+    case DASP_CENT_NOISEFLOOR: c->noiseFloor = (float)v; break;
+    case DASP_CENT_PXLPOWER: c->pxlPower = (float)v; break;
+    case DASP_CENT_CORRTHRESH: c->corrThresh = (float)v; break;
+    case DASP_CENT_SIG: return dasp_cent_update_float(c, code, (double)v);
+    case DASP_CENT_SKYBRIGHTNESS: return dasp_cent_update_float(c, code, (double)v);
+    default: return fail(DASP_CENT_EINVAL, "centmodule: Code %d not recognised, nothing updated", code);
+  }
+  c->tablesDirty = true;
+  return 0;
+}
+
+int dasp_cent_update_float(dasp_cent_t *c, int code, double v) {
+  if (!c) return fail(DASP_CENT_EINVAL, "null handle");
+  switch (code) {
+    case DASP_CENT_SIG:
+      c->sig = (float)v;
+      c->haveSigArr = false;
+      compute_totsig(c);
+      break;
+    case DASP_CENT_READNOISE: c->readnoise = (float)v; break;
+    case DASP_CENT_READBG: c->readbg = (float)v; break;
+    case DASP_CENT_NOISEFLOOR: c->noiseFloor = (float)v; break;
+    case DASP_CENT_SKYBRIGHTNESS:
+      c->skybrightness = (float)v;
+      c->haveSkyArr = false;
+      break;
+    case DASP_CENT_PXLPOWER: c->pxlPower = (float)v; break;
+    case DASP_CENT_CORRTHRESH: c->corrThresh = (float)v; break;
+    default: return dasp_cent_update_int(c, code, (long)v);
+  }
+  c->tablesDirty = true;
+  return 0;
+}
+
+int dasp_cent_update_array(dasp_cent_t *c, int code, const dasp_cent_array *va) {
+  if (!c || !va) return fail(DASP_CENT_EINVAL, "null argument");
+  const dasp_cent_array &a = *va;
+  int rc = 0;
+  switch (code) {
+    case DASP_CENT_SIG:
+      if (!a.data || a.ndim != 1 || a.dims[0] != c->nsubaps) return fail(DASP_CENT_EINVAL, "sigArr should be 1D, size %d", c->nsubaps);
+      c->sigArr.assign((const float *)a.data, (const float *)a.data + c->nsubaps);
+      c->haveSigArr = true;
+      break;
+    case DASP_CENT_SKYBRIGHTNESS:
+      if (!a.data || a.ndim != 3 || a.dims[0] != c->nsubaps || a.dims[1] != c->nimg || a.dims[2] != c->nimg)
+        return fail(DASP_CENT_EINVAL, "Error - centmodule - skybrightness should be 3D array shape nsubaps,nimg,nimg");
+      if (upload(c->d_skyarr, a.data, numel(a) * 4)) return fail(DASP_CENT_EHIP, "upload failed");
+      c->skybrightness = 0.f;
+      c->haveSkyArr = true;
+      break;
+    case DASP_CENT_SPOTPSF:
+      rc = set_spot(c, a);
+      if (rc) return rc;
+      compute_totsig(c);
+      break;
+    case DASP_CENT_CENTWEIGHT:
+      rc = set_cent_weight(c, a);
+      if (rc) return rc;
+      break;
+    case DASP_CENT_CORRPATTERN:
+      if (!a.data) {
+        c->haveCorr = false;
+        break;
+      }
+      if (a.ndim != 4 || c->corrPatternSize != a.dims[2])
+        return fail(DASP_CENT_EINVAL, "Error - centmodule - corrPattern has changed size - please don't do that (or recode)!");
+      rc = set_corr_pattern(c, a);
+      if (rc) return rc;
+      break;
+    case DASP_CENT_REFCENTS:
+      if (!a.data) {
+        c->haveRef = false;
+        break;
+      }
+      if (numel(a) < (long)c->nsubaps * 2) return fail(DASP_CENT_EINVAL, "refCents must hold nsubaps*2 floats");
+      if (upload(c->d_refCents, a.data, (size_t)c->nsubaps * 8)) return fail(DASP_CENT_EHIP, "upload failed");
+      c->haveRef = true;
+      break;
+    case DASP_CENT_CALCOEFF:
+      if (!a.data) {
+        c->haveCalCoeff = false;
+        c->calNCoeff = 0;
+        break;
+      }
+      if (a.ndim != 4 || a.dims[2] != 2 || a.dims[0] * a.dims[1] != c->nsubaps)
+        return fail(DASP_CENT_EINVAL, "Error: centmodule - calCoeff should be 4d (nsubx,nsubx,2,ncoeff)");
+      c->calNCoeff = (int)a.dims[3];
+      if (upload(c->d_calCoeff, a.data, numel(a) * 4)) return fail(DASP_CENT_EHIP, "upload failed");
+      c->haveCalCoeff = true;
+      break;
+    case DASP_CENT_USEBRIGHTEST:
+      if (!a.data || a.ndim != 1 || a.dims[0] != c->nsubaps)
+        return fail(DASP_CENT_EINVAL, "Error: centmodule - useBrightest array should be 1D of shape nsubaps (nsubx^2)");
+      if (upload(c->d_ubArr, a.data, (size_t)c->nsubaps * 4)) return fail(DASP_CENT_EHIP, "upload failed");
+      c->useBrightest = 0;
+      c->haveUBArr = true;
+      break;
+    case DASP_CENT_OUTPUTPHASEARR:
+      break;   // only meaningful with phase interpolation, which the device path rejects
+    default: return fail(DASP_CENT_EINVAL, "centmodule: Code %d not recognised, nothing updated", code);
+  }
+  c->tablesDirty = true;
+  return 0;
+}
+
+int dasp_cent_update_caldata(dasp_cent_t *c, const dasp_cent_array *cd_, const dasp_cent_array *cb,
+                             const dasp_cent_array *cs) {
+  if (!c) return fail(DASP_CENT_EINVAL, "null handle");
+  if (!cd_ || !cd_->data) {
+    c->haveCalData = false;
+    c->calNSteps = 0;
+    return 0;
+  }
+  if (!cb || !cs || !cb->data || !cs->data) return fail(DASP_CENT_EINVAL, "Error: centmodule - calibration stuff not a tuple");
+  if (cs->ndim != 1) return fail(DASP_CENT_EINVAL, "Error: centmodule - calStep should be 1d");
+  const int steps = (int)cs->dims[0];
+  if (cd_->ndim != 4 || cd_->dims[0] != 2 || cd_->dims[1] * cd_->dims[2] != c->nsubaps || cd_->dims[3] != steps)
+    return fail(DASP_CENT_EINVAL, "Error: centmodule - calData should be 4d, (2,nsubx,nsubx,steps)");
+  if (cb->ndim != 4 || cb->dims[0] != 2 || cb->dims[1] * cb->dims[2] != c->nsubaps || cb->dims[3] != 2)
+    return fail(DASP_CENT_EINVAL, "Error: centmodule - calBounds should be 4d (2,nsubx,nsubx,2)");
+  const int *bnd = (const int *)cb->data;
+  for (long q = 0; q < 2L * c->nsubaps; ++q)
+    if (bnd[2 * q] < 0 || bnd[2 * q + 1] < bnd[2 * q] || bnd[2 * q + 1] >= steps)
+      return fail(DASP_CENT_EINVAL, "calBounds entry %ld (%d,%d) outside [0,%d)", q, bnd[2 * q], bnd[2 * q + 1], steps);
+  if (upload(c->d_calData, cd_->data, numel(*cd_) * 4) || upload(c->d_calBounds, cb->data, numel(*cb) * 4) ||
+      upload(c->d_calSteps, cs->data, numel(*cs) * 4))
+    return fail(DASP_CENT_EHIP, "upload failed");
+  c->calNSteps = steps;
+  c->haveCalData = true;
+  return 0;
+}
+
+int dasp_cent_testcorrelation(const float *img, float *corr, int n) {
+  // py_testcorrelation (:3275-3309) on the host, double precision: corr <- HC correlation
+  if (!img || !corr || n <= 0) return fail(DASP_CENT_EINVAL, "bad arguments");
+  std::vector<cd> K((size_t)n * n), X((size_t)n * n);
+  host_hc_to_spectrum(corr, n, n, K.data());
+  for (size_t q = 0; q < X.size(); ++q) X[q] = cd(img[q], 0);
+  host_dft2(X.data(), n, n, -1);
+  for (size_t q = 0; q < X.size(); ++q) X[q] *= K[q];
+  host_dft2(X.data(), n, n, +1);
+  for (size_t q = 0; q < X.size(); ++q) corr[q] = (float)X[q].real();
+  return 0;
+}
+
+}  // extern "C"

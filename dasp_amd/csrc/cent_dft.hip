@@ -1,0 +1,284 @@
+// cent_dft.hip - the DFT-GEMM form on the gfx950 matrix cores.
+//
+// The zero-padded 2-D transform of computeHll (cmod/src/centmodule.c:473-509) for n = 16,
+// F = 32 written as two small GEMMs with the DFT matrix W[k][i] = exp(-2 pi i k i / F):
+//   stage 1 (two subapertures stacked):  Z^T = E^T W^T      (M = 2x16 (s, j), K = 16 (i), N = 32 (k))
+//   stage 2 (per subaperture):           X^T = W  Z^T       (M = 32 (l), K = 16 (j), N = 32 (k))
+// Each complex product is four real ones and each real product is computed from a hi/lo split of
+// both operands into 16-bit floats, hi*hi + hi*lo + lo*hi, accumulated in FP32 by
+// v_mfma_f32_32x32x16_{f16,bf16} (a three-product compensated scheme). W is built in FP32 from
+// exact sincospi values.
+//
+// MFMA 32x32x16 lane layout (checked on the device): A[m][kk] sits in lane m + 32*(kk/8), slot kk%8;
+// B[kk][nn] in lane nn + 32*(kk/8), slot kk%8; D[m][nn] in lane nn + 32*((m/4)%2), register
+// 4*(m/8) + m%4. Stage 1's accumulator therefore holds Z^T[j][k] with lane = k and, for lane half
+// h, j in {4h..4h+3, 8+4h..8+4h+3}: read as a B operand of stage 2 those eight registers are
+// eight K slots, so stage 2 uses W with its K index permuted the same way and needs no transpose.
+// The accumulator of stage 2 holds X[k][l] with lane = k and l = 8g + 4h + u (g, u < 4): |X|^2
+// is binned in-lane along l and across lane pairs along k, then the detector steps and the CoG
+// follow as in the fused FP32 kernel.
+#include "cent_internal.h"
+#include "noise.h"
+
+namespace dasp {
+
+typedef _Float16 h8 __attribute__((ext_vector_type(8)));
+typedef __bf16 b8 __attribute__((ext_vector_type(8)));
+typedef float f16v __attribute__((ext_vector_type(16)));
+
+// hi/lo split of 8 floats into two 16-bit vectors (the residual x - hi is exact in FP32).
+template <typename T8>
+struct Split;
+
+template <>
+struct Split<h8> {
+  static __device__ __forceinline__ void run(const float (&x)[8], h8 &hi, h8 &lo) {
+#pragma unroll
+    for (int t = 0; t < 8; ++t) {
+      const _Float16 hv = (_Float16)x[t];
+      hi[t] = hv;
+      lo[t] = (_Float16)(x[t] - (float)hv);
+    }
+  }
+};
+
+template <>
+struct Split<b8> {
+  static __device__ __forceinline__ void run(const float (&x)[8], b8 &hi, b8 &lo) {
+#pragma unroll
+    for (int t = 0; t < 8; ++t) {
+      const __bf16 hv = (__bf16)x[t];
+      hi[t] = hv;
+      lo[t] = (__bf16)(x[t] - (float)hv);
+    }
+  }
+};
+
+__device__ __forceinline__ f16v mfma(const h8 &a, const h8 &b, const f16v &c) {
+  return __builtin_amdgcn_mfma_f32_32x32x16_f16(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ f16v mfma(const b8 &a, const b8 &b, const f16v &c) {
+  return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0);
+}
+
+// Complex operand split into hi/lo parts of re, im and -im.
+template <typename T8>
+struct COp {
+  T8 rh, rl, ih, il, nih, nil;
+  __device__ __forceinline__ void set(const float (&re)[8], const float (&im)[8]) {
+    float nim[8];
+#pragma unroll
+    for (int t = 0; t < 8; ++t) nim[t] = -im[t];
+    Split<T8>::run(re, rh, rl);
+    Split<T8>::run(im, ih, il);
+    Split<T8>::run(nim, nih, nil);
+  }
+};
+
+// (A.re + i A.im)(B.re + i B.im) accumulated into (cr, ci), three products per real product.
+template <typename T8>
+__device__ __forceinline__ void cgemm(const COp<T8> &A, const COp<T8> &B, f16v &cr, f16v &ci) {
+  cr = mfma(A.rh, B.rh, cr);
+  cr = mfma(A.rh, B.rl, cr);
+  cr = mfma(A.rl, B.rh, cr);
+  cr = mfma(A.ih, B.nih, cr);
+  cr = mfma(A.ih, B.nil, cr);
+  cr = mfma(A.il, B.nih, cr);
+  ci = mfma(A.rh, B.ih, ci);
+  ci = mfma(A.rh, B.il, ci);
+  ci = mfma(A.rl, B.ih, ci);
+  ci = mfma(A.ih, B.rh, ci);
+  ci = mfma(A.ih, B.rl, ci);
+  ci = mfma(A.il, B.rh, ci);
+}
+
+__device__ __forceinline__ float wave_sum(float v) {
+#pragma unroll
+  for (int m = 1; m < 64; m <<= 1) v += __shfl_xor(v, m);
+  return v;
+}
+__device__ __forceinline__ float wave_max(float v) {
+#pragma unroll
+  for (int m = 1; m < 64; m <<= 1) v = fmaxf(v, __shfl_xor(v, m));
+  return v;
+}
+
+constexpr int kN = 16, kF = 32, kM = 16;   // phase size, fft size, detector size (bin 2)
+
+template <typename T8>
+__global__ __launch_bounds__(256) void dft16_cog_kernel(FusedArgs a) {
+  const int l = threadIdx.x & 63;
+  const int h = l >> 5;
+  const int c = l & 31;
+  // constant operands: B1 = W^T (B1[i][k] = W[k][i], lane holds k = c, i = 8h + t) and
+  // A2 = W with permuted K (A2[l][slot q] = W[l][j], j = 8(q/4) + 4h + q%4, lane holds l = c)
+  COp<T8> B1, A2;
+  {
+    float re[8], im[8], re2[8], im2[8];
+#pragma unroll
+    for (int t = 0; t < 8; ++t) {
+      const int i = 8 * h + t;
+      sincospif(-2.0f * (float)((c * i) % kF) / kF, &im[t], &re[t]);
+      const int j = 8 * (t / 4) + 4 * h + t % 4;
+      sincospif(-2.0f * (float)((c * j) % kF) / kF, &im2[t], &re2[t]);
+    }
+    B1.set(re, im);
+    A2.set(re2, im2);
+  }
+  const long npairs = (a.nitems + 1) / 2;
+  const long wave_id = (long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  const long nwaves = (long)gridDim.x * (blockDim.x >> 6);
+  const int n = a.n;
+  for (long pr = wave_id; pr < npairs; pr += nwaves) {
+    // ---- stage 1 operand: lane holds E_s[i][j], s = c/16, j = c%16, i = 8h + t ----
+    const long item1 = 2 * pr + (c >> 4);
+    const bool v1 = item1 < a.nitems;
+    int frame1 = 0, s1 = 0;
+    if (v1) {
+      frame1 = (int)(item1 / a.nactive);
+      s1 = a.active[item1 - (long)frame1 * a.nactive];
+    }
+    const int jj = c & 15;
+    float er[8], ei[8];
+    {
+      const int sy = s1 / a.nsubx, sx = s1 - (s1 / a.nsubx) * a.nsubx;
+      const float *ph = a.phs + (long)frame1 * a.phs_stride + (long)(sy * n) * a.npup + sx * n;
+      const float *pu = a.pup + (long)s1 * n * n;
+#pragma unroll
+      for (int t = 0; t < 8; ++t) {
+        const int i = 8 * h + t;
+        float vr = 0.f, vi = 0.f;
+        if (v1 && i < n && jj < n) {
+          const float th = ph[(long)i * a.npup + jj] - a.tilt[i * n + jj];
+          float sn, cs;
+          sincosf(th, &sn, &cs);
+          const float p = pu[i * n + jj];
+          vr = p * cs;
+          vi = p * sn;
+        }
+        er[t] = vr;
+        ei[t] = vi;
+      }
+    }
+    COp<T8> A1;
+    A1.set(er, ei);
+    f16v zr = {}, zi = {};
+    cgemm(A1, B1, zr, zi);   // Z^T stacked: rows (s, j), columns k
+
+#pragma unroll
+    for (int sub = 0; sub < 2; ++sub) {
+      const long item = 2 * pr + sub;
+      if (item >= a.nitems) break;   // wave-uniform
+      const int frame = (int)(item / a.nactive);
+      const int s = a.active[item - (long)frame * a.nactive];
+      // ---- stage 2: B2 = Z_s^T from accumulator registers 8*sub .. 8*sub+7 ----
+      float br[8], bi[8];
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        br[q] = zr[8 * sub + q];
+        bi[q] = zi[8 * sub + q];
+      }
+      COp<T8> B2;
+      B2.set(br, bi);
+      f16v xr = {}, xi = {};
+      cgemm(A2, B2, xr, xi);   // X^T: rows l, columns k (lane)
+      // ---- |X|^2 binned 2x2: in-lane along l, lane pairs along k ----
+      // register r holds l = 8(r/4) + 4h + r%4; pairs (r, r+1) for even r share a detector column
+      float pix[8];
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        const int r = 2 * q;
+        float v = fmaf(xr[r], xr[r], xi[r] * xi[r]) + fmaf(xr[r + 1], xr[r + 1], xi[r + 1] * xi[r + 1]);
+        v += __shfl_xor(v, 1);
+        pix[q] = v;
+      }
+      const bool owner = (c & 1) == 0;
+      // detector row of the lane pair (k = c, c+1): hll rows F-1-k, F-2-k -> I = (F-2-c)/2
+      const int I = (kF - 2 - c) >> 1;
+      const int M2 = kM * kM;
+      const long pix0 = (long)s * M2;
+      const long noff = (long)frame * a.noise_stride + pix0;
+      const float scale = a.scale[s];
+      const float skyadd = a.skyadd[s];
+      const bool pois = a.poisson_on && a.poisson_on[s];
+      const bool rn = a.readnoise != 0.f && a.zread;
+      float mx = -INFINITY;
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        // l pair starts at l0 = 8(q/2) + 4h + 2(q%2): detector column J = (F-2-l0)/2
+        const int J = (kF - 2 - (8 * (q >> 1) + 4 * h + 2 * (q & 1))) >> 1;
+        float v = pix[q] * scale;
+        if (owner) {
+          const long p = I * kM + J;
+          if (skyadd > 0.f) v += skyadd;
+          if (a.skyarr) v += a.skyarr[pix0 + p];
+          if (pois) v = poisson_from_normal(v, a.zpois[noff + p]);
+          if (a.threshType >= 0 && a.threshType <= 2) v += rn ? fmaf(a.readnoise, a.zread[noff + p], a.readbg) : a.readbg;
+          mx = fmaxf(mx, v);
+        }
+        pix[q] = v;
+      }
+      float thr = a.floor;
+      if (a.threshType == 2) {
+        mx = wave_max(mx);
+        if (a.floor > 0.f && a.floor <= 1.f) thr = a.floor * fmaxf(mx, 0.f);
+      }
+      const int st0 = (kM - a.ncen) / 2, en0 = kM - st0;
+      float sw = 0.f, sxw = 0.f, syw = 0.f;
+      float *bout = a.bimg ? a.bimg + (long)frame * a.bimg_stride + pix0 : nullptr;
+      const float *wt = a.centWeight ? a.centWeight + (a.centWeightDim == 4 ? pix0 : 0) : nullptr;
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        const int J = (kF - 2 - (8 * (q >> 1) + 4 * h + 2 * (q & 1))) >> 1;
+        float v = pix[q];
+        if (a.threshType == 1) v = v < thr ? 0.f : v;
+        else if (a.threshType == 0 || a.threshType == 2) v = v < thr ? 0.f : v - thr;
+        if (a.pxlPower == 2.f) v = v * v;
+        else if (a.pxlPower != 1.f && a.pxlPower != 0.f) v = powf(v, a.pxlPower);
+        if (owner) {
+          if (bout) bout[I * kM + J] = v;
+          if (I >= st0 && I < en0 && J >= st0 && J < en0) {
+            const float tt = wt ? v * wt[I * kM + J] : v;
+            sw += tt;
+            sxw += tt * J;
+            syw += tt * I;
+          }
+        }
+      }
+      sw = wave_sum(sw);
+      sxw = wave_sum(sxw);
+      syw = wave_sum(syw);
+      if (l == 0) {
+        float cx = 0.f, cy = 0.f;
+        if (sw > 0.f) {
+          cx = sxw / sw - (kM - 1) / 2.f;
+          cy = syw / sw - (kM - 1) / 2.f;
+        }
+        if (a.refCents) {
+          cx -= a.refCents[s * 2];
+          cy -= a.refCents[s * 2 + 1];
+        }
+        float *co = a.cents + (long)frame * a.cents_stride + s * 2;
+        co[0] = cx;
+        co[1] = cy;
+      }
+    }
+  }
+}
+
+bool dft_supported(int n, int F, int nimg) { return n == kN && F == kF && nimg == kM; }
+
+hipError_t launch_dft(const FusedArgs &a, int bf16, hipStream_t st) {
+  if (a.nitems <= 0) return hipSuccess;
+  const long npairs = (a.nitems + 1) / 2;
+  long blocks = (npairs + 3) / 4;
+  const long cap = 256L * 8 * 4;   // enough resident waves to fill the chip, then grid-stride
+  if (blocks > cap) blocks = cap;
+  if (bf16)
+    hipLaunchKernelGGL(dft16_cog_kernel<b8>, dim3((unsigned)blocks), dim3(256), 0, st, a);
+  else
+    hipLaunchKernelGGL(dft16_cog_kernel<h8>, dim3((unsigned)blocks), dim3(256), 0, st, a);
+  return hipGetLastError();
+}
+
+}  // namespace dasp

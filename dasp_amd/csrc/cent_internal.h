@@ -1,0 +1,151 @@
+// cent_internal.h - argument blocks shared by the host engine (cent_api.cpp) and the kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace dasp {
+
+constexpr int kMaxSide = 64;  // largest psfsize/clipsize/nimg/corrsize handled on the device
+
+// Image formation: phase -> |FFT|^2 placed as the reference's hll (cmod/src/centmodule.c:473-509).
+struct FormArgs {
+  const float *phs;      // frame 0 phase (npup x npup)
+  const float *amp;      // frame 0 amplitude plane or nullptr
+  long phs_stride;       // floats between frames
+  const float *pup;      // (nsubaps, n, n)
+  const float *tilt;     // (n, n) radians, float as the reference stores it
+  const int *active;     // active subaperture indices
+  int nactive;
+  long nitems;           // nframes * nactive
+  int nsubx, n, npup;
+  int padded;            // paddedsize: side of hll (clip, or psfsize with a spot kernel)
+  float *hll;            // (nitems, padded, padded)
+};
+
+// Everything after image formation, one (frame, subaperture) item per workgroup.
+struct BackArgs {
+  const float *hll;      // (nitems, padded, padded), or nullptr in image-input mode
+  const int *active;
+  int nactive;
+  long nitems;
+  int nsubx, nsubaps, n, F, clip, nimg, ncen, psf, padded, pbf, corrsize, npup;
+  // LGS spot kernel: full Hermitian spectrum (psf x psf complex) per subap or shared
+  const float2 *otf;
+  int otf_per_subap;
+  // binning operator (nimg x clip)
+  const float *binA;
+  // normalisation
+  const float *scale;    // per subap factor sig*frac/totsig (1 where totsig <= 0)
+  const float *totsig;   // per subap theoretical total
+  const float *sigv;     // per subap sig*frac or sigArr
+  const float *pup;      // (nsubaps, n, n): for the phase+amplitude total
+  const float *amp;      // frame 0 amplitude plane or nullptr
+  long phs_stride;
+  // sky
+  const float *skyadd;   // per subap sky*frac (0 when <= 0)
+  const float *skyarr;   // (nsubaps, nimg, nimg) or nullptr
+  // noise (pre-drawn N(0,1), per frame nsubaps*nimg*nimg)
+  const unsigned char *poisson_on;  // per subap flag (addPoisson && !calsource && signal)
+  const float *zpois;
+  const float *zread;
+  long noise_stride;
+  float readnoise;       // already multiplied by (calsource==0)
+  float readbg, floor;
+  int threshType;
+  int opticalBinning;
+  int calsource;
+  int useBrightest;
+  const int *useBrightestArr;
+  float pxlPower;
+  int imageOnly;
+  // correlation
+  int corrMode;
+  float corrThresh;
+  const float2 *corrK;   // (nsubaps, C, C) spectrum for mode 1
+  const float *corrPattern;  // (nsubaps, C, C) raw pattern for modes 2..5
+  // centroid
+  const float *centWeight;
+  int centWeightDim;     // 0, 2 or 4
+  int fitsize;           // effective fit size (3.. ) or -1 for the 3x3 quadratic interpolation
+  int doFit;             // parabolicFit!=0 || gaussianFit>0
+  int gaussianFit;
+  float gaussianMinVal, gaussianReplaceVal;
+  float fitMx[30];
+  const float *calData;  // (2, nsubaps, calNSteps)
+  const int *calBounds;  // (2, nsubaps, 2)
+  const float *calSteps;
+  int calNSteps;
+  const float *calCoeff; // (nsubaps, 2, calNCoeff)
+  int calNCoeff;
+  const float *refCents;
+  // image-input mode (centroidsFromImage, cmod/src/centmodule.c:1987-2127)
+  const float *inputImage;
+  int inputImageDims;    // 0 = phase mode, 2 or 4
+  int inputImagenpxlx;
+  const int *subapLocation;
+  // outputs (frame f adds f*stride)
+  float *cents;
+  long cents_stride;
+  float *bimg;
+  long bimg_stride;
+  float *corrimg;
+  long corr_stride;
+};
+
+// Inactive subapertures: zero slopes, or read-noise-only images when imageOnly != 0.
+struct InactiveArgs {
+  const int *inactive;
+  int ninactive;
+  int nframes;
+  int nsubaps, nimg;
+  int imageOnly;
+  float readnoise, readbg, floor;
+  int threshType;
+  const float *zread;
+  long noise_stride;
+  float *cents;
+  long cents_stride;
+  float *bimg;
+  long bimg_stride;
+};
+
+// Fused fast path (no spot kernel, no prebinning, integer binning from a clip == fftsize image,
+// plain or weighted CoG, or FFT correlation + parabolic fit).
+struct FusedArgs {
+  const float *phs;
+  long phs_stride;
+  const float *pup;      // (nsubaps, n, n)
+  const float *tilt;     // (n, n)
+  const int *active;
+  int nactive;
+  long nitems;
+  int nsubx, n, npup, nimg, ncen;
+  const float *scale;
+  const float *skyadd;
+  const float *skyarr;
+  const unsigned char *poisson_on;
+  const float *zpois;
+  const float *zread;
+  long noise_stride;
+  float readnoise, readbg, floor;
+  int threshType;
+  int calsource;
+  float pxlPower;
+  const float *centWeight;
+  int centWeightDim;
+  const float *refCents;
+  float *cents;
+  long cents_stride;
+  float *bimg;           // may be nullptr (images not returned)
+  long bimg_stride;
+};
+
+hipError_t launch_form(const FormArgs &a, int n_pad, int F, hipStream_t st);
+hipError_t launch_backend(const BackArgs &a, hipStream_t st);
+hipError_t launch_inactive(const InactiveArgs &a, hipStream_t st);
+hipError_t launch_fill(float *p, long n, float v, hipStream_t st);
+bool fused_supported(int n_pad, int F, int nimg);
+hipError_t launch_fused(const FusedArgs &a, int n_pad, int F, hipStream_t st);
+bool dft_supported(int n, int F, int nimg);
+hipError_t launch_dft(const FusedArgs &a, int bf16, hipStream_t st);
+
+}  // namespace dasp

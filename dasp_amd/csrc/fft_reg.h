@@ -1,0 +1,152 @@
+// fft_reg.h - small complex FFTs held in one lane's registers, for CDNA (gfx950) kernels.
+//
+// Forward transform X[k] = sum_i x[i] exp(-2 pi i k i / N), unnormalised (the FFTW_FORWARD
+// convention of the reference, cmod/src/centmodule.c:3105). Radix-2 decimation in time; every
+// twiddle is a compile-time constant and the trivial ones (1, -i, (1-i)/sqrt2, ...) are applied
+// without multiplications. The index permutations are register renames and cost nothing.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <type_traits>
+#include <utility>
+
+namespace dasp {
+
+struct cf {
+  float x, y;
+};
+
+__device__ __forceinline__ cf cadd(cf a, cf b) { return {a.x + b.x, a.y + b.y}; }
+__device__ __forceinline__ cf csub(cf a, cf b) { return {a.x - b.x, a.y - b.y}; }
+__device__ __forceinline__ cf cmul(cf a, cf b) {
+  return {fmaf(a.x, b.x, -a.y * b.y), fmaf(a.x, b.y, a.y * b.x)};
+}
+__device__ __forceinline__ float cabs2(cf a) { return fmaf(a.x, a.x, a.y * a.y); }
+
+// ---- compile-time trigonometry (double Taylor series after reduction to [-pi, pi]) ----------
+namespace ct {
+constexpr double PI = 3.14159265358979323846264338327950288;
+constexpr double reduce(double x) {
+  while (x > PI) x -= 2 * PI;
+  while (x < -PI) x += 2 * PI;
+  return x;
+}
+constexpr double sin(double x) {
+  x = reduce(x);
+  double t = x, s = x;
+  for (int k = 1; k < 40; ++k) {
+    t *= -x * x / ((2.0 * k) * (2.0 * k + 1.0));
+    s += t;
+  }
+  return s;
+}
+constexpr double cos(double x) {
+  x = reduce(x);
+  double t = 1.0, s = 1.0;
+  for (int k = 1; k < 40; ++k) {
+    t *= -x * x / ((2.0 * k - 1.0) * (2.0 * k));
+    s += t;
+  }
+  return s;
+}
+}  // namespace ct
+
+template <int B, int E, typename Fn>
+__device__ __forceinline__ void static_for(Fn &&f) {
+  if constexpr (B < E) {
+    f(std::integral_constant<int, B>{});
+    static_for<B + 1, E>(f);
+  }
+}
+
+// Multiply a by exp(-2 pi i J / M) (J, M compile-time), exploiting the exact special values.
+template <int J, int M>
+__device__ __forceinline__ cf twiddle(cf a) {
+  constexpr int j = ((J % M) + M) % M;
+  if constexpr (j == 0) {
+    return a;
+  } else if constexpr (2 * j == M) {
+    return {-a.x, -a.y};
+  } else if constexpr (4 * j == M) {  // -i
+    return {a.y, -a.x};
+  } else if constexpr (4 * j == 3 * M) {  // +i
+    return {-a.y, a.x};
+  } else if constexpr (8 * j == M) {  // (1-i)/sqrt2
+    constexpr float s = 0.70710678118654752440f;
+    return {(a.x + a.y) * s, (a.y - a.x) * s};
+  } else if constexpr (8 * j == 3 * M) {  // (-1-i)/sqrt2
+    constexpr float s = 0.70710678118654752440f;
+    return {(a.y - a.x) * s, -(a.x + a.y) * s};
+  } else if constexpr (8 * j == 5 * M) {  // (-1+i)/sqrt2
+    constexpr float s = 0.70710678118654752440f;
+    return {-(a.x + a.y) * s, (a.x - a.y) * s};
+  } else if constexpr (8 * j == 7 * M) {  // (1+i)/sqrt2
+    constexpr float s = 0.70710678118654752440f;
+    return {(a.x - a.y) * s, (a.x + a.y) * s};
+  } else {
+    constexpr float c = (float)ct::cos(-2.0 * ct::PI * j / M);
+    constexpr float s = (float)ct::sin(-2.0 * ct::PI * j / M);
+    return cmul(a, cf{c, s});
+  }
+}
+
+template <int N>
+constexpr int ilog2() {
+  int l = 0;
+  while ((1 << l) < N) ++l;
+  return l;
+}
+template <int BITS>
+constexpr int bitrev(int v) {
+  int r = 0;
+  for (int b = 0; b < BITS; ++b) r |= ((v >> b) & 1) << (BITS - 1 - b);
+  return r;
+}
+
+// In-place forward FFT of length N (power of two, N <= 64) on x[0..N-1].
+template <int N>
+__device__ __forceinline__ void fft_fwd(cf (&x)[N]) {
+  static_assert((N & (N - 1)) == 0 && N >= 1 && N <= 128, "power-of-two length");
+  if constexpr (N == 1) {
+    return;
+  } else {
+    constexpr int L = ilog2<N>();
+    cf y[N];
+    static_for<0, N>([&](auto i) { y[bitrev<L>(decltype(i)::value)] = x[decltype(i)::value]; });
+    static_for<1, L + 1>([&](auto s) {
+      constexpr int m = 1 << decltype(s)::value;
+      constexpr int h = m / 2;
+      static_for<0, N / m>([&](auto g) {
+        constexpr int k = decltype(g)::value * m;
+        static_for<0, h>([&](auto jj) {
+          constexpr int j = decltype(jj)::value;
+          cf t = twiddle<j, m>(y[k + j + h]);
+          cf u = y[k + j];
+          y[k + j] = cadd(u, t);
+          y[k + j + h] = csub(u, t);
+        });
+      });
+    });
+    static_for<0, N>([&](auto i) { x[decltype(i)::value] = y[decltype(i)::value]; });
+  }
+}
+
+// Zero-padded transform: x holds NP inputs (x[i], i<NP; the rest of the F-point input is zero);
+// out[k] = sum_{i<NP} x[i] exp(-2 pi i k i / F) for k < F, with F = R*NP.
+// out[R*q + r] = FFT_NP(x[i] * w_F^{i r})[q].
+template <int NP, int F>
+__device__ __forceinline__ void fft_fwd_padded(const cf (&x)[NP], cf (&out)[F]) {
+  constexpr int R = F / NP;
+  static_assert(R * NP == F && R >= 1, "F must be a multiple of NP");
+  static_for<0, R>([&](auto rr) {
+    constexpr int r = decltype(rr)::value;
+    cf y[NP];
+    static_for<0, NP>([&](auto ii) {
+      constexpr int i = decltype(ii)::value;
+      y[i] = twiddle<i * r, F>(x[i]);
+    });
+    fft_fwd<NP>(y);
+    static_for<0, NP>([&](auto q) { out[R * decltype(q)::value + r] = y[decltype(q)::value]; });
+  });
+}
+
+}  // namespace dasp

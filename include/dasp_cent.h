@@ -1,0 +1,189 @@
+/*
+ * dasp_cent.h - C ABI of the MI355X (gfx950) Shack-Hartmann image-formation and centroiding
+ * engine. It replaces the CPython module cmod.cent of DASP (cmod/src/centmodule.c) behind the
+ * util.centcmod.centcmod construct/run/update contract (util/centcmod.py:41-145).
+ *
+ * Conventions
+ *   - Every entry point returns 0 on success and a negative DASP_CENT_E* code on failure; the
+ *     message of the last failure on the calling thread is returned by dasp_cent_last_error().
+ *   - "host" pointers are borrowed, exactly as the reference borrows numpy buffers
+ *     (cmod/src/centmodule.c:2978-3000): the caller keeps them alive until dasp_cent_free() or
+ *     until it replaces them through dasp_cent_update_*(). Outputs (cents, bimg, corrimg) are
+ *     written in place by dasp_cent_run().
+ *   - "device" pointers are HIP device addresses; stream arguments are hipStream_t passed as
+ *     void* (NULL = the legacy default stream).
+ *   - Arrays are C-contiguous float32 / int32 in the reference layouts.
+ */
+#ifndef DASP_CENT_H
+#define DASP_CENT_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DASP_CENT_ABI_VERSION 1
+
+/* error codes */
+#define DASP_CENT_OK 0
+#define DASP_CENT_EINVAL (-1)      /* bad argument, shape or type */
+#define DASP_CENT_EUNSUPPORTED (-2) /* valid for the reference, not supported by this build */
+#define DASP_CENT_EHIP (-3)        /* HIP runtime error */
+#define DASP_CENT_ENOMEM (-4)
+#define DASP_CENT_ENODEV (-5)      /* no gfx950 device */
+
+/* update codes: identical values to cmod/src/centmodule.c:109-130 and util/centcmod.py:19-40 */
+#define DASP_CENT_CALSOURCE 1
+#define DASP_CENT_SIG 2
+#define DASP_CENT_ADDPOISSON 3
+#define DASP_CENT_NCEN 4
+#define DASP_CENT_READNOISE 5
+#define DASP_CENT_READBG 6
+#define DASP_CENT_NOISEFLOOR 7
+#define DASP_CENT_SKYBRIGHTNESS 8
+#define DASP_CENT_PXLPOWER 9
+#define DASP_CENT_SPOTPSF 10
+#define DASP_CENT_NTHREADS 11
+#define DASP_CENT_OPTICALBINNING 12
+#define DASP_CENT_CENTWEIGHT 13
+#define DASP_CENT_CORRELATIONCENTROIDING 14
+#define DASP_CENT_CORRTHRESH 15
+#define DASP_CENT_CORRPATTERN 16
+#define DASP_CENT_CALDATA 17
+#define DASP_CENT_REFCENTS 18
+#define DASP_CENT_CALCOEFF 19
+#define DASP_CENT_USEBRIGHTEST 20
+#define DASP_CENT_INTEGSTEPS 21
+#define DASP_CENT_OUTPUTPHASEARR 22
+
+/* compute forms (extension; the reference has only its FFTW path) */
+#define DASP_CENT_FORM_FFT32 0     /* fused FP32 FFT in registers/LDS */
+#define DASP_CENT_FORM_DFT_BF16 1  /* DFT-matrix GEMMs on MFMA, BF16 hi/lo, FP32 accumulate */
+#define DASP_CENT_FORM_DFT_FP16 2  /* DFT-matrix GEMMs on MFMA, FP16 hi/lo, FP32 accumulate */
+#define DASP_CENT_FORM_FFT32_GENERAL 3 /* FP32 FFT through the general (unfused) kernels only */
+
+typedef struct dasp_cent dasp_cent_t;
+
+/* An n-dimensional float32/int32 array argument; data==NULL stands for Python's None. */
+typedef struct {
+  void *data;
+  int ndim;
+  long dims[4];
+} dasp_cent_array;
+
+/*
+ * The 41 arguments of cent.initialise in the reference order
+ * (cmod/src/centmodule.c:2876-2883, format "iiiiiiiffifOOifiilOO!OO!O!O!O!iOifOOiiOiiiffOO";
+ * caller util/centcmod.py:115). Arguments that Python passes as "float or array" have both forms:
+ * the array form wins when its data pointer is non-NULL.
+ */
+typedef struct {
+  int nthreads;              /* host threads of the reference; ignored by the GPU engine */
+  int nsubaps;               /* nsubx*nsubx */
+  int ncen, fftsize, clipsize, nimg, phasesize;
+  float readnoise, readbg;
+  int addPoisson;
+  float noiseFloor;
+  float sig;                 /* used when sigArr.data == NULL */
+  dasp_cent_array sigArr;    /* f32 (nsubaps,) */
+  float skybrightness;       /* used when skybrightnessArr.data == NULL */
+  dasp_cent_array skybrightnessArr; /* f32 (nsubaps,nimg,nimg) */
+  int calsource;
+  float pxlPower;
+  int nintegrations, nlatency;
+  unsigned long seed;
+  dasp_cent_array phs;       /* f32 (npup,npup) phase or (2,npup,npup) phase+amplitude; NULL = image input */
+  dasp_cent_array pupfn;     /* f32 (nsubx,nsubx,n,n) */
+  dasp_cent_array spotpsf;   /* f32 (P,P) or (nsubx,nsubx,P,P), P>=clipsize, or NULL */
+  dasp_cent_array cents;     /* f32 (nsubx,nsubx,2) output */
+  dasp_cent_array subflag;   /* i32 (nsubx,nsubx) */
+  dasp_cent_array bimg;      /* f32 (nsubx,nsubx,nimg,nimg) output */
+  dasp_cent_array fracSubArea; /* f32 (nsubx,nsubx) */
+  int opticalBinning;
+  dasp_cent_array centWeight;  /* f32 (corrsize,corrsize) or (nsubx,nsubx,corrsize,corrsize) or NULL */
+  int correlationCentroiding;
+  float corrThresh;
+  dasp_cent_array corrPattern; /* f32 (nsubx,nsubx,C,C) in FFTW half-complex form (util/correlation.py:20-51) */
+  dasp_cent_array corrimg;     /* f32 (nsubx,nsubx,C,C) output */
+  int threshType, imageOnly;
+  int useBrightest;            /* used when useBrightestArr.data == NULL */
+  dasp_cent_array useBrightestArr; /* i32 (nsubaps,) */
+  int preBinningFactor, parabolicFit, gaussianFit;
+  float gaussianMinVal, gaussianReplaceVal;
+  dasp_cent_array inputImage;  /* f32 2-D (nsubx*nimg)^2 or 4-D (nsubx,nsubx,nimg,nimg), or NULL */
+  dasp_cent_array subapLocation; /* i32 nsubaps*4, or NULL */
+} dasp_cent_params;
+
+/* Message of the last failure on this thread ("" if none). */
+const char *dasp_cent_last_error(void);
+/* DASP_CENT_ABI_VERSION of the loaded library. */
+int dasp_cent_abi_version(void);
+
+/* cent.initialise (cmod/src/centmodule.c:2858-3161). Validates like the reference, uploads the
+ * per-subaperture tables to the current HIP device and returns an opaque handle. */
+int dasp_cent_initialise(const dasp_cent_params *p, dasp_cent_t **out);
+
+/* cent.run (cmod/src/centmodule.c:3188-3229): phase -> images -> centroids for one simulation
+ * step, including the nintegrations/nlatency cycle (:1774-1788, :3221-3223). Reads the borrowed
+ * phs, writes cents/bimg/corrimg in place; *seconds receives the wall time like py_run. */
+int dasp_cent_run(dasp_cent_t *h, float *seconds);
+
+/* cent.runSlope (cmod/src/centmodule.c:3230-3272): detector image -> centroids. */
+int dasp_cent_run_slope(dasp_cent_t *h, float *seconds);
+
+/* cent.update (cmod/src/centmodule.c:2573-2856), split by the Python type of the new value.
+ * _int: CALSOURCE, ADDPOISSON, NCEN, NTHREADS, OPTICALBINNING, CORRELATIONCENTROIDING,
+ *       USEBRIGHTEST (scalar), INTEGSTEPS.
+ * _float: SIG (scalar), READNOISE, READBG, NOISEFLOOR, SKYBRIGHTNESS (scalar), PXLPOWER, CORRTHRESH.
+ * _array: SIG (array), SKYBRIGHTNESS (array), SPOTPSF, CENTWEIGHT, CORRPATTERN, REFCENTS, CALCOEFF,
+ *         USEBRIGHTEST (array), OUTPUTPHASEARR; a NULL data pointer stands for None.
+ * _caldata: CALDATA with (calData f32 (2,nsubx,nsubx,steps), calBounds i32 (2,nsubx,nsubx,2),
+ *           calSteps f32 (steps,)); all NULL stands for None (:2153-2203). */
+int dasp_cent_update_int(dasp_cent_t *h, int code, long value);
+int dasp_cent_update_float(dasp_cent_t *h, int code, double value);
+int dasp_cent_update_array(dasp_cent_t *h, int code, const dasp_cent_array *value);
+int dasp_cent_update_caldata(dasp_cent_t *h, const dasp_cent_array *calData,
+                             const dasp_cent_array *calBounds, const dasp_cent_array *calSteps);
+
+/* cent.free (cmod/src/centmodule.c:3163-3186). */
+int dasp_cent_free(dasp_cent_t *h);
+
+/* cent.testcorrelation (cmod/src/centmodule.c:3275-3309): corr (n,n), a half-complex transformed
+ * reference, is replaced by the correlation of img (n,n) with it. Host arrays. */
+int dasp_cent_testcorrelation(const float *img, float *corr, int n);
+
+/* ---- extensions (no counterpart in the reference) ---------------------------------------- */
+
+/* Select the compute form (DASP_CENT_FORM_*). */
+int dasp_cent_set_form(dasp_cent_t *h, int form);
+
+/* Device-resident step over nframes phase frames (science/wfscent.py:630-702 keeps phase,
+ * images and slopes in host memory; here they stay in HBM). Frame f reads
+ * d_phs + f*phs_stride and writes d_cents + f*nsubaps*2, d_bimg + f*nsubaps*nimg^2 (if non-NULL)
+ * and d_corrimg + f*nsubaps*C^2 (if non-NULL). d_zpois/d_zread are pre-drawn N(0,1) arrays of
+ * nsubaps*nimg^2 per frame (may be NULL when the noise they feed is off). Asynchronous on stream.
+ * Every frame is a readout (nintegrations+nlatency must be 1). */
+int dasp_cent_run_device(dasp_cent_t *h, const float *d_phs, long phs_stride, int nframes,
+                         float *d_cents, float *d_bimg, float *d_corrimg,
+                         const float *d_zpois, const float *d_zread, void *stream);
+
+/* Host noise generator used by dasp_cent_run: n standard normals from (seed, stream), PCG32
+ * (O'Neill 2014) uniforms through the Box-Muller transform. Deterministic and host only. */
+int dasp_cent_draw_normals(unsigned long seed, unsigned long stream, long n, float *out);
+
+/* Complex spectrum (interleaved re,im; C*C per pattern) equivalent to a half-complex correlation
+ * pattern, i.e. the K with calcCorrelation(img) = IDFT(DFT(pad(img)) * K) (host only). */
+int dasp_cent_hc_to_spectrum(const float *pattern_hc, int C, float *spectrum);
+
+/* Number of active subapertures, phase grid width and correlation image size of a handle. */
+int dasp_cent_info(dasp_cent_t *h, int *nactive, int *npup, int *corrsize);
+
+/* Kernel timing: when enabled, every later launch of the handle's main image kernel is bracketed
+ * by HIP events on the launch stream. dasp_cent_kernel_time returns the summed milliseconds and
+ * the number of launches measured since the last reset (it synchronises the last event). */
+int dasp_cent_set_timing(dasp_cent_t *h, int enable);
+int dasp_cent_kernel_time(dasp_cent_t *h, int reset, double *ms, long *launches);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* DASP_CENT_H */

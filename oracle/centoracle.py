@@ -332,6 +332,8 @@ def read_ccd(img, zread, readbg, readnoise, floor, thresh_type):
     1: x<f?0:x; 2 (extension, not in the reference): the floor is f*max(x) when 0<f<=1
     (thresholdCorrelation semantics, :1488-1505), then subtract-and-clip."""
     x = np.asarray(img, f64)
+    if thresh_type not in (0, 1, 2):
+        return x   # the reference prints "Unknown threshold type" and leaves the data (:797, :813)
     if readnoise == 0 or zread is None:
         x = x + readbg
     else:
@@ -348,7 +350,6 @@ def read_ccd(img, zread, readbg, readnoise, floor, thresh_type):
             thr = np.full(S, floor)
         thr = thr.reshape((S,) + (1,) * (x.ndim - 1))
         return np.where(x < thr, 0.0, x - thr)
-    return x   # reference: prints "Unknown threshold type" and leaves the data
 
 
 def select_brightest(img, use_brightest):
@@ -1076,7 +1077,7 @@ def run_phase(p, phs, zpois=None, zread=None, bimg_in=None):
     return cents, bimg, corrimg
 
 
-def run_image(p, image, bimg_in=None, subap_location=None):
+def run_image(p, image, bimg_in=None, subap_location=None, zread=None):
     """centroidsFromImage, cmod/src/centmodule.c:1987-2127: image (nsubx*nimg)^2 2-D or
     (nsubx,nsubx,nimg,nimg) 4-D."""
     S = p.nsubaps
@@ -1091,7 +1092,9 @@ def run_image(p, image, bimg_in=None, subap_location=None):
     for s in range(S):
         if p.subflag[s] != 1:
             if p.imageOnly:
-                bimg[s] = read_ccd(np.zeros((1, nimg, nimg)), None, p.readbg, 0.0,
+                rn = p.readnoise * (p.calsource == 0)
+                z = None if zread is None else np.asarray(zread[s], f64)[None]
+                bimg[s] = read_ccd(np.zeros((1, nimg, nimg)), z, p.readbg, rn,
                                    p.noiseFloor, p.threshType)[0]
             continue
         x = bimg[s].copy()

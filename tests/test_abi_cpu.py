@@ -1,0 +1,87 @@
+"""C-ABI checks that need no GPU: the library loads, exports every symbol include/dasp_cent.h
+declares, its host-only helpers agree with the oracle, and GPU entry points fail loudly here."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+from oracle import centoracle as O
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _lib():
+    from dasp_amd.cmod import cent
+    return cent, cent.library()
+
+
+def test_exports_every_declared_symbol():
+    _, lib = _lib()
+    hdr = open(os.path.join(ROOT, "include", "dasp_cent.h")).read()
+    names = sorted(set(re.findall(r"\b(dasp_cent_\w+)\s*\(", hdr)))
+    assert len(names) >= 16
+    for nm in names:
+        assert hasattr(lib, nm), nm
+    assert lib.dasp_cent_abi_version() == 1
+
+
+def test_noise_stream_is_deterministic_normal():
+    cent, _ = _lib()
+    a = cent.drawNormals(7, 3, 200001)
+    b = cent.drawNormals(7, 3, 200001)
+    c = cent.drawNormals(7, 4, 1000)
+    np.testing.assert_array_equal(a, b)
+    assert not np.array_equal(a[:1000], c)
+    assert abs(a.mean()) < 0.01 and abs(a.std() - 1) < 0.01
+    assert abs((a ** 3).mean()) < 0.03 and abs((a ** 4).mean() - 3) < 0.05
+
+
+@pytest.mark.parametrize("C", [4, 6, 7, 8, 16, 32])
+def test_hc_to_spectrum_matches_oracle(C):
+    cent, _ = _lib()
+    rng = np.random.default_rng(C)
+    hc = O.transform_psf(rng.random((C, C)))
+    K = cent.hcToSpectrum(hc)
+    K2 = O.hc_pattern_to_spectrum(hc.astype(np.float64))
+    np.testing.assert_allclose(K, K2, atol=2e-6 * np.abs(K2).max())
+
+
+@pytest.mark.parametrize("N", [8, 16])
+def test_testcorrelation_matches_oracle(N):
+    cent, _ = _lib()
+    rng = np.random.default_rng(N)
+    img = rng.random((N, N)).astype(np.float32)
+    ref = rng.random((N, N))
+    corr = O.transform_psf(ref)
+    want = O.calc_correlation_hc(img.astype(np.float64), corr.astype(np.float64), N, N)
+    cent.testcorrelation(img, corr)
+    np.testing.assert_allclose(corr, want, rtol=1e-5, atol=1e-4)
+
+
+def test_initialise_rejects_bad_arrays_before_touching_the_gpu():
+    cent, _ = _lib()
+    nsubx, n = 2, 8
+    pupfn = np.ones((nsubx, nsubx, n, n), np.float32)
+    cents = np.zeros((nsubx, nsubx, 2), np.float64)        # wrong dtype
+    with pytest.raises(cent.error):
+        cent.initialise(1, 4, 8, 16, 16, 8, n, 0., 0., 0, 0., 1.0, 0., 0, 1., 1, 0, 1,
+                        np.zeros((16, 16), np.float32), pupfn, None, cents, np.ones((2, 2), np.int32),
+                        np.zeros((2, 2, 8, 8), np.float32), np.ones((2, 2), np.float32), 0, None, 0, 0.,
+                        None, None, 0, 0, 0, 1, 0, 0, 0., 0., None, None)
+
+
+def test_no_silent_cpu_fallback_without_gpu():
+    """With no usable gfx950 device the engine refuses to initialise; it never computes on the CPU."""
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is present")
+    cent, _ = _lib()
+    nsubx, n = 2, 8
+    with pytest.raises(cent.error):
+        cent.initialise(1, 4, 8, 16, 16, 8, n, 0., 0., 0, 0., 1.0, 0., 0, 1., 1, 0, 1,
+                        np.zeros((16, 16), np.float32), np.ones((2, 2, n, n), np.float32), None,
+                        np.zeros((2, 2, 2), np.float32), np.ones((2, 2), np.int32),
+                        np.zeros((2, 2, 8, 8), np.float32), np.ones((2, 2), np.float32), 0, None, 0, 0.,
+                        None, None, 0, 0, 0, 1, 0, 0, 0., 0., None, None)

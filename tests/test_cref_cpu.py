@@ -1,0 +1,67 @@
+"""The FP32 C restatement (oracle/cref) against the FP64 oracle, on the CPU."""
+import numpy as np
+import pytest
+
+from dasp_amd import workloads as W
+from dasp_amd.util.correlation import transformPSF
+from oracle import centoracle as O
+from oracle.cref import CRef
+
+
+def _geo(w):
+    pup = w.pupil()
+    return (pup.perSubap(w.nsubx, 0).reshape(-1, w.n, w.n), pup.getSubapFlag(w.nsubx, 0.5),
+            (pup.getSubarea(w.nsubx) / w.n ** 2).astype(np.float32))
+
+
+@pytest.mark.parametrize("nthreads", [1, 3])
+def test_cref_noiseless_cog(nthreads):
+    w = W.WFS("t", 20, 16, 32, 16, 8.0, 0.2, 0.15, 25.0)
+    pupfn, fl, fr = _geo(w)
+    ph = w.phase()
+    c, b, _ = CRef(w.nsubx, w.n, w.nfft, w.clip, w.nimg, w.cen, pupfn, fl, fr).run(ph, nthreads=nthreads)
+    oc, ob, _ = O.run_phase(O.Params(w.nsubx, w.n, w.nfft, w.clip, w.nimg, w.cen, pupfn, fl, fr), ph)
+    act = fl.ravel() == 1
+    assert np.abs(c[act] - oc[act]).max() < 1e-4
+    assert np.all(c[~act] == 0)
+
+
+def test_cref_noise_same_normals():
+    w = W.WFS("t", 12, 16, 32, 16, 8.0, 0.14, 0.12, 25.0)
+    pupfn, fl, fr = _geo(w)
+    S = w.nsubx ** 2
+    rng = np.random.default_rng(4)
+    zp = rng.standard_normal((S, 16, 16)).astype(np.float32)
+    zr = rng.standard_normal((S, 16, 16)).astype(np.float32)
+    kw = dict(sig=1000.0, addPoisson=1, readnoise=0.5, noiseFloor=0.1, threshType=2)
+    ph = w.phase()
+    c, b, _ = CRef(w.nsubx, w.n, w.nfft, w.clip, w.nimg, w.cen, pupfn, fl, fr, **kw).run(ph, zp, zr)
+    oc, ob, _ = O.run_phase(O.Params(w.nsubx, w.n, w.nfft, w.clip, w.nimg, w.cen, pupfn, fl, fr, **kw), ph, zp, zr)
+    act = fl.ravel() == 1
+    diff = np.abs(b[act] - ob[act]) > 1e-3 * np.maximum(1, np.abs(ob[act]))
+    assert diff.sum() <= 3
+    clean = ~diff.reshape(diff.shape[0], -1).any(axis=1)
+    assert np.abs(c[act][clean] - oc[act][clean]).max() < 1e-4
+
+
+def test_cref_correlation_fit_and_spot():
+    w = W.WFS("t", 8, 16, 32, 16, 8.0, 0.2, 0.15, 25.0, spot=True)
+    pupfn, fl, fr = _geo(w)
+    ph = w.phase()
+    rng = np.random.default_rng(5)
+    ref = np.exp(-((np.arange(16) - 7.5) ** 2)[:, None] / 6 - ((np.arange(16) - 7.5) ** 2)[None] / 6)
+    ref = np.broadcast_to(ref, (8, 8, 16, 16)).astype(np.float32)
+    hc = transformPSF(ref)
+    kw = dict(corrPattern=hc, parabolicFit=5)
+    c, _, corr = CRef(w.nsubx, w.n, w.nfft, w.clip, w.nimg, w.cen, pupfn, fl, fr, **kw).run(ph)
+    p = O.Params(w.nsubx, w.n, w.nfft, w.clip, w.nimg, w.cen, pupfn, fl, fr, correlationCentroiding=1,
+                 corrPattern=hc, parabolicFit=5)
+    oc, _, ocorr = O.run_phase(p, ph)
+    act = fl.ravel() == 1
+    assert np.abs(c[act] - oc[act]).max() < 1e-4
+    spots = w.spots()
+    c, _, _ = CRef(w.nsubx, w.n, w.nfft, w.clip, w.nimg, w.cen, pupfn, fl, fr, sig=1000.,
+                   spotpsf=spots.reshape(-1, 32, 32)).run(ph)
+    oc, _, _ = O.run_phase(O.Params(w.nsubx, w.n, w.nfft, w.clip, w.nimg, w.cen, pupfn, fl, fr, sig=1000.,
+                                    spotpsf=spots.reshape(-1, 32, 32)), ph)
+    assert np.abs(c[act] - oc[act]).max() < 1e-4

@@ -1,0 +1,244 @@
+"""GPU parity: the gfx950 engine, called through the drop-in centcmod contract, against the FP64
+oracle on the same phase and the same pre-drawn noise.
+
+Bar: noiseless centroids within 1e-4 px absolute (FP32 engine vs FP64 oracle). With thresholds
+or noise, a pixel can sit on a decision boundary (floor, Poisson CDF step) where FP32 and FP64
+round differently; those tests bound the number of differing pixels and check the rest to 1e-4.
+"""
+import numpy as np
+import pytest
+
+from dasp_amd import workloads as W
+from oracle import centoracle as O
+from tests._cases import Case, report, say
+
+pytestmark = pytest.mark.gpu
+
+
+def _phase(w, frame=0):
+    return W.WFS.phase(w, frame)
+
+
+def _active(case):
+    return case.cc.subflag.ravel() == 1
+
+
+PATHS = {"fused": 0, "general": 3}
+
+
+@pytest.mark.parametrize("path", sorted(PATHS))
+@pytest.mark.parametrize("wfs", [W.C1, W.C3], ids=["c1_8x8", "c3_80x80"])
+def test_noiseless_cog_vs_oracle(wfs, path):
+    say("case noiseless CoG %s %s" % (wfs.name, path))
+    c = Case(wfs.nsubx, wfs.n, wfs.nfft, wfs.nimg, obs=wfs.obs, phs=wfs.phase(), form=PATHS[path])
+    oc, ob, _ = c.run_both()
+    act = _active(c)
+    d = report(wfs.name, c.cents, oc, act)
+    assert d.max() < 1e-4
+    assert np.all(c.cents.reshape(-1, 2)[~act] == 0)
+    b = c.bimg.reshape(-1, wfs.nimg, wfs.nimg)[act]
+    np.testing.assert_allclose(b, ob[act], rtol=0, atol=2e-5 * ob[act].max())
+    np.testing.assert_allclose(b.sum(axis=(1, 2)), ob[act].sum(axis=(1, 2)), rtol=2e-5)
+    c.free()
+
+
+def test_flat_phase_reference_and_refcents():
+    say("case flat phase + REFCENTS")
+    w = W.C1
+    c = Case(w.nsubx, w.n, w.nfft, w.nimg, obs=w.obs)
+    c.run_both()
+    ref = c.cents.copy()
+    assert np.abs(ref).max() < 1e-5          # symmetric pupil, centred spot
+    c.phs[:] = w.phase()
+    c.cc.update(18, ref)                     # REFCENTS
+    c.params.refCents = ref
+    oc, _, _ = c.run_both()
+    assert report("refcents", c.cents, oc, _active(c)).max() < 1e-4
+    c.free()
+
+
+@pytest.mark.parametrize("path", sorted(PATHS))
+def test_noise_poisson_read_relative_floor(path):
+    """40x40 WFS, 1000 photons, Poisson + read noise (sigma 0.5) and a 0.1*max floor."""
+    w = W.C2
+    say("case noise %s %s" % (w.name, path))
+    c = Case(w.nsubx, w.n, w.nfft, w.nimg, obs=w.obs, sig=w.sig, readnoise=w.readnoise, addPoisson=1,
+             noiseFloor=w.noiseFloor, threshType=2, phs=w.phase(), seed=5, form=PATHS[path])
+    oc, ob, _ = c.run_both()
+    act = _active(c)
+    b = c.bimg.reshape(-1, w.nimg, w.nimg)[act]
+    diffpix = np.abs(b - ob[act]) > 1e-3 * np.maximum(1, np.abs(ob[act]))
+    say("noisy pixels differing: %d of %d" % (diffpix.sum(), diffpix.size))
+    assert diffpix.sum() <= max(3, 1e-4 * diffpix.size)
+    clean = ~diffpix.reshape(diffpix.shape[0], -1).any(axis=1)
+    d = np.abs(c.cents.reshape(-1, 2)[act] - oc[act])
+    say("subaps without a flipped pixel: %d of %d; their max|d|=%.3g px" % (clean.sum(), act.sum(), d[clean].max()))
+    assert d[clean].max() < 1e-4
+    # the noisy images are integers plus read noise: check the photon count statistics
+    assert abs(b.sum() / act.sum() - ob[act].sum() / act.sum()) < 5
+    c.free()
+
+
+def test_correlation_parabolic_fit():
+    """Nyquist-sampled n=32/nfft=64/nimg=16 WFS, FFT correlation with the flat-phase image as
+    the reference (util/centroid.py:1580-1626), 5x5 parabolic fit."""
+    say("case correlation + parabolic fit")
+    nsubx = 12
+    wgeo = W.WFS("corr", nsubx, 32, 64, 16, 8.0, 0.2, 0.15, 25.0)
+    ref = W.reference_images(wgeo)
+    c = Case(nsubx, 32, 64, 16, obs=0.2, correlationCentroiding=1, corrPattern=ref, parabolicFit=5,
+             phs=wgeo.phase())
+    oc, ob, ocorr = c.run_both()
+    act = _active(c)
+    d = report("corr+fit", c.cents, oc, act)
+    assert d.max() < 1e-4
+    ci = c.corrimg.reshape(-1, 16, 16)[act]
+    np.testing.assert_allclose(ci, ocorr[act], rtol=0, atol=1e-5 * np.abs(ocorr[act]).max())
+    c.free()
+
+
+def test_lgs_spot_convolution():
+    say("case LGS spot convolution")
+    wgeo = W.WFS("lgs", 10, 16, 32, 16, 8.0, 0.2, 0.15, 25.0, spot=True, sig=1000.0)
+    spots = wgeo.spots()
+    c = Case(10, 16, 32, 16, obs=0.2, sig=1000.0, spotpsf=spots, phs=wgeo.phase())
+    oc, ob, _ = c.run_both()
+    d = report("lgs", c.cents, oc, _active(c))
+    assert d.max() < 1e-4
+    c.free()
+
+
+OPTION_CASES = {
+    "pxlpower2": dict(pxlPower=2.),
+    "pxlpower1.5": dict(pxlPower=1.5),
+    "brightest12": dict(useBrightest=12),
+    "brightest-12": dict(useBrightest=-12),
+    "sky_scalar_floor0": dict(skybrightness=0.01, noiseFloor=0.02, threshType=0),
+    "floor_type1": dict(noiseFloor=0.01, threshType=1),
+    "ncen6": dict(ncen=6),
+    "clip_fractional_bin": dict(clip=14, nimg=6),
+    "optical_binning": dict(opticalBinning=1),
+    "imageOnly": dict(imageOnly=1),
+    "sigarr": "sigarr",
+    "skyarr": "skyarr",
+    "centweight4d": "centweight",
+    "brute_corr5": "brute",
+    "diffcorr2": "diff2",
+    "diffcorr4": "diff4",
+    "gaussfit": "gauss",
+    "quadinterp": "quad",
+}
+
+
+@pytest.mark.parametrize("path", sorted(PATHS))
+@pytest.mark.parametrize("name", sorted(OPTION_CASES))
+def test_options(name, path):
+    say("case option %s %s" % (name, path))
+    nsubx, n, nfft, nimg = 6, 8, 16, 8
+    wgeo = W.WFS("opt", nsubx, n, nfft, nimg, 4.2, 0.25, 0.2, 30.0)
+    phs = wgeo.phase() * 0.5
+    S = nsubx * nsubx
+    kw = OPTION_CASES[name]
+    rng = np.random.default_rng(3)
+    if kw == "sigarr":
+        kw = dict(sig=(rng.random(S) * 2 + 0.5).astype(np.float32))
+    elif kw == "skyarr":
+        kw = dict(skybrightness=(rng.random((S, nimg, nimg)) * 0.01).astype(np.float32))
+    elif kw == "centweight":
+        kw = dict(centWeight=(rng.random((nsubx, nsubx, nimg, nimg)) + 0.5).astype(np.float32))
+    elif kw in ("brute", "diff2", "diff4", "gauss", "quad"):
+        ref = W.reference_images(wgeo)
+        mode = {"brute": 5, "diff2": 2, "diff4": 4, "gauss": 1, "quad": 1}[kw]
+        kw = dict(correlationCentroiding=mode, corrPattern=ref.astype(np.float32))
+        if name == "gaussfit":
+            kw.update(gaussianFitVals=(1e-3, -10.))
+        if name == "quadinterp":
+            kw.update(parabolicFit=-1)
+    kw = dict(kw)
+    nimg_ = kw.pop("nimg", nimg)
+    c = Case(nsubx, n, nfft, nimg_, obs=0.25, phs=phs, form=PATHS[path], **kw)
+    oc, ob, ocorr = c.run_both()
+    act = _active(c)
+    b = c.bimg.reshape(-1, nimg_, nimg_)[act]
+    np.testing.assert_allclose(b, ob[act], rtol=0, atol=1e-4 * max(1e-30, np.abs(ob[act]).max()))
+    if not c.params.imageOnly:
+        d = report(name, c.cents, oc, act)
+        assert d.max() < 1e-4
+    c.free()
+
+
+def test_integration_cycle():
+    """nintegrations=2, nlatency=1: a readout every third call (cmod/src/centmodule.c:1774-1788)."""
+    say("case integration cycle")
+    w = W.C1
+    c = Case(w.nsubx, w.n, w.nfft, w.nimg, obs=w.obs, nintegrations=2, nlatency=1, phs=w.phase())
+    outs = []
+    for k in range(6):
+        c.cents[:] = -7
+        c.cc.run(0)
+        outs.append(c.cents.copy())
+    ro = [np.all(o != -7) for o in outs]
+    assert ro == [False, False, True, False, False, True]
+    c.free()
+
+
+def test_image_input_runslope():
+    say("case runSlope image input")
+    w = W.C1
+    src = Case(w.nsubx, w.n, w.nfft, w.nimg, obs=w.obs, phs=w.phase(), sig=100.)
+    src.cc.run(0)
+    img4 = src.bimg.copy()
+    img2 = np.ascontiguousarray(img4.transpose(0, 2, 1, 3).reshape(w.nsubx * w.nimg, w.nsubx * w.nimg))
+    src.free()
+    for inp in (img2, img4):
+        c = Case(w.nsubx, w.n, w.nfft, w.nimg, obs=w.obs, noiseFloor=0.05, inputImage=inp)
+        before = c.bimg.copy()
+        c.cc.runSlope(0)
+        oc, ob, _ = O.run_image(c.params, inp, bimg_in=before)
+        assert report("runSlope%dd" % inp.ndim, c.cents, oc, _active(c)).max() < 1e-4
+        c.free()
+
+
+def test_run_device_frames_match_legacy():
+    """Device-resident multi-frame step equals per-frame legacy runs."""
+    import torch
+    from dasp_amd.cmod import cent
+    say("case run_device frames")
+    w = W.C1
+    c = Case(w.nsubx, w.n, w.nfft, w.nimg, obs=w.obs)
+    frames = np.stack([w.phase(f) for f in range(3)])
+    want = []
+    for f in range(3):
+        c.phs[:] = frames[f]
+        c.cc.run(0)
+        want.append(c.cents.copy())
+    dev = torch.device("cuda:0")
+    ph = torch.from_numpy(frames).to(dev)
+    S = w.nsubx * w.nsubx
+    ce = torch.zeros((3, S, 2), dtype=torch.float32, device=dev)
+    bi = torch.zeros((3, S, w.nimg, w.nimg), dtype=torch.float32, device=dev)
+    cent.runDevice(c.cc.centstruct, ph.data_ptr(), w.npup * w.npup, 3, ce.data_ptr(), bi.data_ptr(),
+                   stream=torch.cuda.current_stream().cuda_stream)
+    torch.cuda.synchronize()
+    got = ce.cpu().numpy().reshape(3, w.nsubx, w.nsubx, 2)
+    np.testing.assert_array_equal(got, np.stack(want))
+    c.free()
+
+
+@pytest.mark.parametrize("form", ["fp16", "bf16"])
+def test_dft_mfma_form_error_vs_fp64(form):
+    """The DFT-GEMM form (hi/lo split, FP32 accumulate) on the 80x80 nfft=32 WFS: RMS centroid
+    error against FP64 within 1e-3 px (FP16 split is expected near the FP32 path)."""
+    from dasp_amd.cmod import cent
+    w = W.C3
+    say("case DFT form %s" % form)
+    c = Case(w.nsubx, w.n, w.nfft, w.nimg, obs=w.obs, phs=w.phase(),
+             form={"fp16": cent.FORM_DFT_FP16, "bf16": cent.FORM_DFT_BF16}[form])
+    oc, ob, _ = c.run_both()
+    act = _active(c)
+    d = report("dft-" + form, c.cents, oc, act)
+    rms = np.sqrt((d ** 2).mean())
+    assert rms < 1e-3
+    if form == "fp16":
+        assert d.max() < 1e-4
+    c.free()
