@@ -55,6 +55,13 @@ def library():
     if _lib is None:
         if not os.path.exists(_LIBPATH):
             raise ImportError("libdasp_cent.so not built (run python -m dasp_amd.build): %s" % _LIBPATH)
+        # One HIP runtime per process: PyTorch ships its own libamdhip64 (same soname). Loading it
+        # first makes this library bind to it, so torch tensors and the engine share the device
+        # context; loading the system runtime first would leave torch unable to initialise HIP.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         lib = ctypes.CDLL(_LIBPATH)
         vp, ip, fp = ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_float)
         lib.dasp_cent_last_error.restype = ctypes.c_char_p

@@ -156,6 +156,7 @@ def prepare_spot_otf(spotpsf):
     sp = np.asarray(spotpsf, f64)
     if sp.ndim == 2:
         return np.fft.rfft2(flip_array(sp, 1))
+    sp = sp.reshape(-1, sp.shape[-2], sp.shape[-1])
     return np.stack([np.fft.rfft2(flip_array(s, 1)) for s in sp])
 
 
