@@ -81,12 +81,14 @@ def library():
         lib.dasp_cent_hc_to_spectrum.argtypes = [vp, ip, vp]
         lib.dasp_cent_info.argtypes = [vp, ctypes.POINTER(ip), ctypes.POINTER(ip), ctypes.POINTER(ip)]
         lib.dasp_cent_set_timing.argtypes = [vp, ip]
+        lib.dasp_cent_set_fast_trig.argtypes = [vp, ip]
         lib.dasp_cent_kernel_time.argtypes = [vp, ip, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_long)]
         for name in ("dasp_cent_initialise", "dasp_cent_run", "dasp_cent_run_slope", "dasp_cent_update_int",
                      "dasp_cent_update_float", "dasp_cent_update_array", "dasp_cent_update_caldata",
                      "dasp_cent_free", "dasp_cent_testcorrelation", "dasp_cent_set_form",
                      "dasp_cent_run_device", "dasp_cent_draw_normals", "dasp_cent_hc_to_spectrum",
-                     "dasp_cent_info", "dasp_cent_set_timing", "dasp_cent_kernel_time"):
+                     "dasp_cent_info", "dasp_cent_set_timing", "dasp_cent_kernel_time",
+                     "dasp_cent_set_fast_trig"):
             getattr(lib, name).restype = ctypes.c_int
         _lib = lib
     return _lib
@@ -279,6 +281,11 @@ def setForm(handle, form):
     """Choose the compute form: FORM_FFT32 (default; fused kernel where the options allow),
     FORM_DFT_BF16, FORM_DFT_FP16, or FORM_FFT32_GENERAL (FP32, general kernels only)."""
     _check(library().dasp_cent_set_form(_ptr(handle), int(form)))
+
+
+def setFastTrig(handle, enable=True):
+    """Use the reduced-argument hardware sine/cosine for exp(i(phi - t)) in the fused kernels."""
+    _check(library().dasp_cent_set_fast_trig(_ptr(handle), int(bool(enable))))
 
 
 def info(handle):

@@ -96,6 +96,7 @@ struct dasp_cent {
   int corrsize = 0, corrPatternSize = 0, npup = 0;
   int atmosPhaseType = 0, interpolate = 0;
   int curInteg = 0, form = DASP_CENT_FORM_FFT32;
+  int fastTrig = 0;
   unsigned long seed = 1;
   unsigned long nreadouts = 0;
   float sig = 0.f, skybrightness = 0.f, readnoise = 0.f, readbg = 0.f, noiseFloor = 0.f;
@@ -441,6 +442,8 @@ static int launch_phase(dasp_cent *c, const float *d_phs, long phs_stride, int n
     fa.phs_stride = phs_stride;
     fa.pup = c->d_pup.as<float>();
     fa.tilt = c->d_tilt.as<float>();
+    fa.tiltstep = (float)(M_PI / c->F * (c->F + 1.));
+    fa.fast_trig = c->fastTrig;
     fa.active = c->d_active.as<int>();
     fa.nactive = (int)c->active.size();
     fa.nitems = nitems;
@@ -469,6 +472,9 @@ static int launch_phase(dasp_cent *c, const float *d_phs, long phs_stride, int n
     fa.cents_stride = (long)c->nsubaps * 2;
     fa.bimg = d_bimg;
     fa.bimg_stride = (long)c->nsubaps * c->nimg * c->nimg;
+    fa.slow_detector = (d_zpois && need_pois(c)) || (d_zread && need_read(c)) || c->haveSkyArr ||
+                       c->threshType == 2 || (c->pxlPower != 1.f && c->pxlPower != 0.f) || c->haveCentWeight ||
+                       d_bimg != nullptr;
     int trc = timing_mark(c, st);
     if (trc) return trc;
     if (use_dft(c)) HIPCHK(launch_dft(fa, c->form == DASP_CENT_FORM_DFT_BF16, st));
@@ -797,6 +803,12 @@ int dasp_cent_set_form(dasp_cent_t *c, int form) {
     return fail(DASP_CENT_EUNSUPPORTED, "the DFT-GEMM form covers n=16, fftsize=clipsize=32, nimg=16 with a (weighted) CoG, "
                                         "no spot kernel, phase only");
   c->form = form;
+  return 0;
+}
+
+int dasp_cent_set_fast_trig(dasp_cent_t *c, int enable) {
+  if (!c) return fail(DASP_CENT_EINVAL, "null handle");
+  c->fastTrig = enable != 0;
   return 0;
 }
 

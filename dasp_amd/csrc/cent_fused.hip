@@ -18,6 +18,10 @@
 #include "fft_reg.h"
 #include "noise.h"
 
+#ifndef DASP_FUSED_WPE
+#define DASP_FUSED_WPE
+#endif
+
 namespace dasp {
 
 template <int NP, int F, int B>
@@ -57,16 +61,30 @@ __device__ __forceinline__ float group_max(float v) {
   return v;
 }
 
-template <int NP, int F, int B>
-__global__ __launch_bounds__(256) void fused_cog_kernel(FusedArgs a) {
-  using S = FusedShape<NP, F, B>;
-  constexpr int SPW = S::SPW, R = S::R, NIMG = S::NIMG, RB = S::RB, G = S::G, TS = S::TS;
-  __shared__ cf stage[S::WAVES][SPW * NP * TS];
-  const int lane = threadIdx.x & 63;
-  const int w = threadIdx.x >> 6;
-  const int sl = lane / NP;
-  const int j = lane - sl * NP;   // pass 1: phase column; pass 2: row group m
-  const long item = ((long)blockIdx.x * S::WAVES + w) * SPW + sl;
+// exp(i th) for the phase argument. Accurate path: sincosf. Fast path: Cody-Waite reduction by
+// 2 pi (two-constant split) to |rem| <= pi, then the hardware sine/cosine of rem/(2 pi) revolutions.
+template <bool FAST>
+__device__ __forceinline__ void phase_sincos(float th, float *sn, float *cs) {
+  if constexpr (FAST) {
+    const float k = rintf(th * 0.15915494309189535f);
+    float rem = fmaf(-k, 6.28318548202514648f, th);
+    rem = fmaf(-k, -1.7484556000744883e-07f, rem);
+    const float r = rem * 0.15915494309189535f;
+    *sn = __builtin_amdgcn_sinf(r);
+    *cs = __builtin_amdgcn_cosf(r);
+  } else {
+    sincosf(th, sn, cs);
+  }
+}
+
+template <int NP>
+struct PhaseCol {
+  float ph[NP], pu[NP];
+};
+
+// Load phase column j (rows i < n) and the pupil of one work item; zeros where out of range.
+template <int NP>
+__device__ __forceinline__ void load_item(const FusedArgs &a, long item, int j, PhaseCol<NP> &c) {
   const bool valid = item < a.nitems;
   int frame = 0, s = 0;
   if (valid) {
@@ -77,20 +95,35 @@ __global__ __launch_bounds__(256) void fused_cog_kernel(FusedArgs a) {
   const int sy = s / a.nsubx, sx = s - (s / a.nsubx) * a.nsubx;
   const float *ph = a.phs + (long)frame * a.phs_stride + (long)(sy * n) * a.npup + sx * n;
   const float *pu = a.pup + (long)s * n * n;
+  static_for<0, NP>([&](auto ii) {
+    constexpr int i = decltype(ii)::value;
+    const bool in = valid && i < n && j < n;
+    c.ph[i] = in ? ph[(long)i * a.npup + j] : 0.f;
+    c.pu[i] = in ? pu[i * n + j] : 0.f;
+  });
+}
 
-  // ---- pass 1: columns ----
+// One work item (frame, subaperture) per NP-lane group: image formation, detector, CoG.
+template <int NP, int F, int B, bool FAST>
+__device__ __forceinline__ void fused_item(const FusedArgs &a, long item, int j, int w, int sl,
+                                           const PhaseCol<NP> &col, cf *st) {
+  using S = FusedShape<NP, F, B>;
+  constexpr int R = S::R, NIMG = S::NIMG, RB = S::RB, G = S::G, TS = S::TS;
+  const bool valid = item < a.nitems;
+  int frame = 0, s = 0;
+  if (valid) {
+    frame = (int)(item / a.nactive);
+    s = a.active[item - (long)frame * a.nactive];
+  }
+  const int n = a.n;
+  // ---- pass 1 operand: E[i][j] = pup * exp(i (phi - t)) ----
   cf x[NP];
   static_for<0, NP>([&](auto ii) {
     constexpr int i = decltype(ii)::value;
-    cf v = {0.f, 0.f};
-    if (valid && i < n && j < n) {
-      const float th = ph[(long)i * a.npup + j] - a.tilt[i * n + j];
-      float sn, cs;
-      sincosf(th, &sn, &cs);
-      const float p = pu[i * n + j];
-      v = {p * cs, p * sn};
-    }
-    x[i] = v;
+    const float th = col.ph[i] - a.tiltstep * (float)(i + j + 1 - n);   // float ops as the reference
+    float sn, cs;
+    phase_sincos<FAST>(th, &sn, &cs);
+    x[i] = {col.pu[i] * cs, col.pu[i] * sn};
   });
 
   // ---- transpose in R phases + pass 2 + binning ----
@@ -100,7 +133,6 @@ __global__ __launch_bounds__(256) void fused_cog_kernel(FusedArgs a) {
   static_for<0, RB>([&](auto ib) {
     static_for<0, NIMG>([&](auto jb) { acc[decltype(ib)::value][decltype(jb)::value] = 0.f; });
   });
-  cf *st = &stage[w][sl * NP * TS];
   static_for<0, R>([&](auto rr) {
     constexpr int r = decltype(rr)::value;
     {
@@ -144,48 +176,70 @@ __global__ __launch_bounds__(256) void fused_cog_kernel(FusedArgs a) {
   // detector row of acc[ib]: lane (row group j) holds rows k = j*R .. j*R+R-1
   auto det_row = [&](int ib) { return R >= B ? (F - j * R) / B - 1 - ib : (F - 1 - j * R) / B; };
 
-  // ---- detector: normalise, sky, noise, threshold, power ----
+  // ---- detector: normalise, sky, noise, threshold, power, CoG window sums ----
   const int M2 = NIMG * NIMG;
   const long pix0 = (long)s * M2;
-  const long noff = (long)frame * a.noise_stride + pix0;
   const float scale = valid ? a.scale[s] : 1.f;
   const float skyadd = valid ? a.skyadd[s] : 0.f;
-  const bool pois = valid && a.poisson_on && a.poisson_on[s];
-  const bool rn = a.readnoise != 0.f && a.zread;
-  float mx = -INFINITY;
-  static_for<0, RB>([&](auto ibb) {
-    constexpr int ib = decltype(ibb)::value;
-    const int I = det_row(ib);
-    static_for<0, NIMG>([&](auto jbb) {
-      constexpr int jb = decltype(jbb)::value;
-      float v = acc[ib][jb];
+  const int st0 = (NIMG - a.ncen) / 2, en0 = NIMG - st0;
+  float sw = 0.f, sxw = 0.f, syw = 0.f;
+  if (!a.slow_detector) {
+    // no noise, sky array, relative floor, power, weights or image output: all inline
+    const float add = (skyadd > 0.f ? skyadd : 0.f) + (a.threshType >= 0 && a.threshType <= 2 ? a.readbg : 0.f);
+    const float thr = a.floor;
+    static_for<0, RB>([&](auto ibb) {
+      constexpr int ib = decltype(ibb)::value;
+      const int I = det_row(ib);
+      static_for<0, NIMG>([&](auto jbb) {
+        constexpr int jb = decltype(jbb)::value;
+        float v = fmaf(acc[ib][jb], scale, 0.f) + add;
+        if (a.threshType == 1) v = v < thr ? 0.f : v;
+        else if (a.threshType == 0) v = v < thr ? 0.f : v - thr;
+        if (owner && I >= st0 && I < en0 && jb >= st0 && jb < en0) {
+          sw += v;
+          sxw += v * jb;
+          syw += v * I;
+        }
+      });
+    });
+  } else {
+    // general detector: stage the lane's pixels in its LDS slot and walk them with a
+    // runtime loop, so that the rarely used per-pixel code is emitted once
+    float *px = reinterpret_cast<float *>(st) + j * (RB * NIMG);
+    static_for<0, RB>([&](auto ib) {
+      static_for<0, NIMG>([&](auto jb) {
+        px[decltype(ib)::value * NIMG + decltype(jb)::value] = acc[decltype(ib)::value][decltype(jb)::value];
+      });
+    });
+    const long noff = (long)frame * a.noise_stride + pix0;
+    const bool pois = valid && a.poisson_on && a.poisson_on[s];
+    const bool rn = a.readnoise != 0.f && a.zread;
+    float mx = -INFINITY;
+#pragma unroll 1
+    for (int q = 0; q < RB * NIMG; ++q) {
+      const int I = det_row(q / NIMG), jb = q % NIMG;
+      float v = px[q] * scale;   // scale is 1 where the theoretical total is not positive
       if (owner) {
         const long p = I * NIMG + jb;
-        v *= scale;   // 1 where the theoretical total is not positive
         if (skyadd > 0.f) v += skyadd;
         if (a.skyarr) v += a.skyarr[pix0 + p];
         if (pois) v = poisson_from_normal(v, a.zpois[noff + p]);
         if (a.threshType >= 0 && a.threshType <= 2) v += rn ? fmaf(a.readnoise, a.zread[noff + p], a.readbg) : a.readbg;
         mx = fmaxf(mx, v);
       }
-      acc[ib][jb] = v;
-    });
-  });
-  float thr = a.floor;
-  if (a.threshType == 2) {
-    mx = group_max<NP>(mx);
-    if (a.floor > 0.f && a.floor <= 1.f) thr = a.floor * fmaxf(mx, 0.f);
-  }
-  const int st0 = (NIMG - a.ncen) / 2, en0 = NIMG - st0;
-  float sw = 0.f, sxw = 0.f, syw = 0.f;
-  float *bout = a.bimg ? a.bimg + (long)frame * a.bimg_stride + pix0 : nullptr;
-  const float *wt = a.centWeight ? a.centWeight + (a.centWeightDim == 4 ? pix0 : 0) : nullptr;
-  static_for<0, RB>([&](auto ibb) {
-    constexpr int ib = decltype(ibb)::value;
-    const int I = det_row(ib);
-    static_for<0, NIMG>([&](auto jbb) {
-      constexpr int jb = decltype(jbb)::value;
-      float v = acc[ib][jb];
+      px[q] = v;
+    }
+    float thr = a.floor;
+    if (a.threshType == 2) {
+      mx = group_max<NP>(mx);
+      if (a.floor > 0.f && a.floor <= 1.f) thr = a.floor * fmaxf(mx, 0.f);
+    }
+    float *bout = a.bimg ? a.bimg + (long)frame * a.bimg_stride + pix0 : nullptr;
+    const float *wt = a.centWeight ? a.centWeight + (a.centWeightDim == 4 ? pix0 : 0) : nullptr;
+#pragma unroll 1
+    for (int q = 0; q < RB * NIMG; ++q) {
+      const int I = det_row(q / NIMG), jb = q % NIMG;
+      float v = px[q];
       if (a.threshType == 1) v = v < thr ? 0.f : v;
       else if (a.threshType == 0 || a.threshType == 2) v = v < thr ? 0.f : v - thr;
       if (a.pxlPower == 2.f) v = v * v;
@@ -199,8 +253,9 @@ __global__ __launch_bounds__(256) void fused_cog_kernel(FusedArgs a) {
           syw += t * I;
         }
       }
-    });
-  });
+    }
+    wave_lds_sync();   // the slot is the next item's transpose buffer
+  }
   sw = group_sum<NP>(sw);
   sxw = group_sum<NP>(sxw);
   syw = group_sum<NP>(syw);
@@ -220,13 +275,45 @@ __global__ __launch_bounds__(256) void fused_cog_kernel(FusedArgs a) {
   }
 }
 
+// Persistent waves: wave w handles item groups w, w + nwaves, ... and loads the next group's
+// phase and pupil while it transforms the current one.
+template <int NP, int F, int B, bool FAST>
+__global__ __launch_bounds__(256) DASP_FUSED_WPE void fused_cog_kernel(FusedArgs a) {
+  using S = FusedShape<NP, F, B>;
+  constexpr int SPW = S::SPW, TS = S::TS;
+  __shared__ cf stage[S::WAVES][SPW * NP * TS];
+  const int lane = threadIdx.x & 63;
+  const int w = threadIdx.x >> 6;
+  const int sl = lane / NP;
+  const int j = lane - sl * NP;   // pass 1: phase column; pass 2: row group m
+  const long ngroups = (a.nitems + SPW - 1) / SPW;
+  const long stride = (long)gridDim.x * S::WAVES;
+  long g = (long)blockIdx.x * S::WAVES + w;
+  if (g >= ngroups) return;       // wave-uniform
+  cf *st = &stage[w][sl * NP * TS];
+  PhaseCol<NP> cur;
+  load_item<NP>(a, g * SPW + sl, j, cur);
+  for (; g < ngroups; g += stride) {
+    PhaseCol<NP> nxt;
+    load_item<NP>(a, (g + stride) * SPW + sl, j, nxt);   // items past the end load zeros
+    fused_item<NP, F, B, FAST>(a, g * SPW + sl, j, w, sl, cur, st);
+    cur = nxt;
+  }
+}
+
 template <int NP, int F, int B>
 static hipError_t launch_fused_t(const FusedArgs &a, hipStream_t st) {
   using S = FusedShape<NP, F, B>;
   const long per_block = (long)S::WAVES * S::SPW;
-  const long blocks = (a.nitems + per_block - 1) / per_block;
+  long blocks = (a.nitems + per_block - 1) / per_block;
   if (blocks <= 0) return hipSuccess;
-  hipLaunchKernelGGL((fused_cog_kernel<NP, F, B>), dim3((unsigned)blocks), dim3(64 * S::WAVES), 0, st, a);
+  // persistent grid: enough workgroups for every CU to hold several, then grid-stride
+  const long cap = 256L * 4;
+  if (blocks > cap) blocks = cap;
+  if (a.fast_trig)
+    hipLaunchKernelGGL((fused_cog_kernel<NP, F, B, true>), dim3((unsigned)blocks), dim3(64 * S::WAVES), 0, st, a);
+  else
+    hipLaunchKernelGGL((fused_cog_kernel<NP, F, B, false>), dim3((unsigned)blocks), dim3(64 * S::WAVES), 0, st, a);
   return hipGetLastError();
 }
 

@@ -115,6 +115,9 @@ struct FusedArgs {
   long phs_stride;
   const float *pup;      // (nsubaps, n, n)
   const float *tilt;     // (n, n)
+  float tiltstep;        // the reference's float tmp = pi/F*(F+1): tilt[i][j] = tmp*(i+j+1-n)
+  int fast_trig;         // 1: Cody-Waite reduction + hardware sin/cos instead of sincosf
+  int slow_detector;     // any of noise, sky array, relative floor, power, weights, image output
   const int *active;
   int nactive;
   long nitems;

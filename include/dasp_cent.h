@@ -156,6 +156,10 @@ int dasp_cent_testcorrelation(const float *img, float *corr, int n);
 /* Select the compute form (DASP_CENT_FORM_*). */
 int dasp_cent_set_form(dasp_cent_t *h, int form);
 
+/* Phase factor exp(i(phi - t)) in the fused kernels: 0 = sincosf (default), 1 = two-constant
+ * Cody-Waite reduction by 2 pi and the hardware sine/cosine (v_sin_f32/v_cos_f32). */
+int dasp_cent_set_fast_trig(dasp_cent_t *h, int enable);
+
 /* Device-resident step over nframes phase frames (science/wfscent.py:630-702 keeps phase,
  * images and slopes in host memory; here they stay in HBM). Frame f reads
  * d_phs + f*phs_stride and writes d_cents + f*nsubaps*2, d_bimg + f*nsubaps*nimg^2 (if non-NULL)

@@ -23,7 +23,7 @@ def _active(case):
     return case.cc.subflag.ravel() == 1
 
 
-PATHS = {"fused": 0, "general": 3}
+PATHS = {"fused": 0, "general": 3, "fused_fasttrig": 10}
 
 
 @pytest.mark.parametrize("path", sorted(PATHS))
