@@ -73,7 +73,7 @@ def main():
     ap.add_argument("--workload", default="c3", choices=["c1", "c2", "c3", "c5"])
     ap.add_argument("--frames", type=int, default=0, help="frames per step (default: the workload's)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--fast-trig", action="store_true", help="hardware sin/cos after Cody-Waite reduction")
+    ap.add_argument("--accurate-trig", action="store_true", help="sincosf instead of the reduced hardware sin/cos")
     ap.add_argument("--calib-mib", type=int, default=0,
                     help="after timing, copy a buffer of this many MiB once (known-size kernel for scaling "
                          "rocprofv3 FETCH_SIZE/WRITE_SIZE counters)")
@@ -108,7 +108,7 @@ def main():
                            None)
     h = cc.centstruct
     cent.setForm(h, {"fft32": 0, "bf16": 1, "fp16": 2}[args.form])
-    cent.setFastTrig(h, args.fast_trig)
+    cent.setFastTrig(h, not args.accurate_trig)
     nact, npup, csz = cent.info(h)
 
     # synthetic phase frames, generated and kept in HBM
@@ -175,7 +175,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": w.name, "nsubx": w.nsubx, "n": w.n, "nfft": w.nfft, "nimg": w.nimg,
                    "npup": w.npup, "frames_per_step": frames, "active_subaps": nact, "form": args.form,
-                   "noise": noise, "images_returned": False, "fast_trig": args.fast_trig, "wall_s": wall},
+                   "noise": noise, "images_returned": False, "fast_trig": not args.accurate_trig, "wall_s": wall},
     }
     if klaunch > 0:
         kavg = kms / klaunch / 1e3

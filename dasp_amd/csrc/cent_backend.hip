@@ -718,9 +718,23 @@ __global__ void inactive_kernel(InactiveArgs a) {
   }
 }
 
+// Zero slopes of inactive subapertures: one thread per (frame, subaperture).
+__global__ void inactive_cents_kernel(InactiveArgs a) {
+  const long total = (long)a.nframes * a.ninactive;
+  const long it = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (it >= total) return;
+  const int frame = (int)(it / a.ninactive);
+  const int s = a.inactive[it - (long)frame * a.ninactive];
+  *reinterpret_cast<float2 *>(a.cents + (long)frame * a.cents_stride + s * 2) = make_float2(0.f, 0.f);
+}
+
 hipError_t launch_inactive(const InactiveArgs &a, hipStream_t st) {
   const long total = (long)a.nframes * a.ninactive;
   if (total <= 0) return hipSuccess;
+  if (a.imageOnly == 0) {
+    hipLaunchKernelGGL(inactive_cents_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, a);
+    return hipGetLastError();
+  }
   long blocks = total < 65535 ? total : 65535;
   hipLaunchKernelGGL(inactive_kernel, dim3((unsigned)blocks), dim3(256), 0, st, a);
   return hipGetLastError();

@@ -61,9 +61,20 @@ __device__ __forceinline__ f16v mfma(const b8 &a, const b8 &b, const f16v &c) {
   return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0);
 }
 
-// Complex operand split into hi/lo parts of re, im and -im.
+// Complex operand split into hi/lo parts of re and im (per work item).
 template <typename T8>
 struct COp {
+  T8 rh, rl, ih, il;
+  __device__ __forceinline__ void set(const float (&re)[8], const float (&im)[8]) {
+    Split<T8>::run(re, rh, rl);
+    Split<T8>::run(im, ih, il);
+  }
+};
+
+// Constant DFT operand: also carries -im, so that the subtraction in re*re - im*im needs no
+// per-item negated copy.
+template <typename T8>
+struct CConst {
   T8 rh, rl, ih, il, nih, nil;
   __device__ __forceinline__ void set(const float (&re)[8], const float (&im)[8]) {
     float nim[8];
@@ -76,19 +87,37 @@ struct COp {
 };
 
 // (A.re + i A.im)(B.re + i B.im) accumulated into (cr, ci), three products per real product.
+// Stage 1: A varies per item, B = W^T is constant (negation on B).
 template <typename T8>
-__device__ __forceinline__ void cgemm(const COp<T8> &A, const COp<T8> &B, f16v &cr, f16v &ci) {
+__device__ __forceinline__ void cgemm_vc(const COp<T8> &A, const CConst<T8> &B, f16v &cr, f16v &ci) {
   cr = mfma(A.rh, B.rh, cr);
-  cr = mfma(A.rh, B.rl, cr);
-  cr = mfma(A.rl, B.rh, cr);
-  cr = mfma(A.ih, B.nih, cr);
-  cr = mfma(A.ih, B.nil, cr);
-  cr = mfma(A.il, B.nih, cr);
   ci = mfma(A.rh, B.ih, ci);
+  cr = mfma(A.rh, B.rl, cr);
   ci = mfma(A.rh, B.il, ci);
+  cr = mfma(A.rl, B.rh, cr);
   ci = mfma(A.rl, B.ih, ci);
+  cr = mfma(A.ih, B.nih, cr);
   ci = mfma(A.ih, B.rh, ci);
+  cr = mfma(A.ih, B.nil, cr);
   ci = mfma(A.ih, B.rl, ci);
+  cr = mfma(A.il, B.nih, cr);
+  ci = mfma(A.il, B.rh, ci);
+}
+
+// Stage 2: A = W is constant (negation on A), B varies per item.
+template <typename T8>
+__device__ __forceinline__ void cgemm_cv(const CConst<T8> &A, const COp<T8> &B, f16v &cr, f16v &ci) {
+  cr = mfma(A.rh, B.rh, cr);
+  ci = mfma(A.rh, B.ih, ci);
+  cr = mfma(A.rh, B.rl, cr);
+  ci = mfma(A.rh, B.il, ci);
+  cr = mfma(A.rl, B.rh, cr);
+  ci = mfma(A.rl, B.ih, ci);
+  cr = mfma(A.nih, B.ih, cr);
+  ci = mfma(A.ih, B.rh, ci);
+  cr = mfma(A.nih, B.il, cr);
+  ci = mfma(A.ih, B.rl, ci);
+  cr = mfma(A.nil, B.ih, cr);
   ci = mfma(A.il, B.rh, ci);
 }
 
@@ -105,14 +134,30 @@ __device__ __forceinline__ float wave_max(float v) {
 
 constexpr int kN = 16, kF = 32, kM = 16;   // phase size, fft size, detector size (bin 2)
 
-template <typename T8>
+template <bool FAST>
+__device__ __forceinline__ void dft_sincos(float th, float *sn, float *cs) {
+  if constexpr (FAST) {
+    const float k = rintf(th * 0.15915494309189535f);
+    float rem = fmaf(-k, 6.28318548202514648f, th);
+    rem = fmaf(-k, -1.7484556000744883e-07f, rem);
+    const float r = rem * 0.15915494309189535f;
+    *sn = __builtin_amdgcn_sinf(r);
+    *cs = __builtin_amdgcn_cosf(r);
+  } else {
+    sincosf(th, sn, cs);
+  }
+}
+
+template <typename T8, bool FAST>
 __global__ __launch_bounds__(256) void dft16_cog_kernel(FusedArgs a) {
+  __shared__ float pxbuf[4][64 * 8];
   const int l = threadIdx.x & 63;
   const int h = l >> 5;
   const int c = l & 31;
+  float *px = &pxbuf[threadIdx.x >> 6][l * 8];
   // constant operands: B1 = W^T (B1[i][k] = W[k][i], lane holds k = c, i = 8h + t) and
   // A2 = W with permuted K (A2[l][slot q] = W[l][j], j = 8(q/4) + 4h + q%4, lane holds l = c)
-  COp<T8> B1, A2;
+  CConst<T8> B1, A2;
   {
     float re[8], im[8], re2[8], im2[8];
 #pragma unroll
@@ -129,41 +174,51 @@ __global__ __launch_bounds__(256) void dft16_cog_kernel(FusedArgs a) {
   const long wave_id = (long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
   const long nwaves = (long)gridDim.x * (blockDim.x >> 6);
   const int n = a.n;
-  for (long pr = wave_id; pr < npairs; pr += nwaves) {
-    // ---- stage 1 operand: lane holds E_s[i][j], s = c/16, j = c%16, i = 8h + t ----
-    const long item1 = 2 * pr + (c >> 4);
-    const bool v1 = item1 < a.nitems;
-    int frame1 = 0, s1 = 0;
-    if (v1) {
-      frame1 = (int)(item1 / a.nactive);
-      s1 = a.active[item1 - (long)frame1 * a.nactive];
+  const int jj = c & 15;
+  // stage-1 operand of a pair: lane holds E_s[i][j], s = c/16, j = c%16, i = 8h + t
+  auto load_pair = [&](long pr, float (&ph8)[8], float (&pu8)[8]) {
+    const long it = 2 * pr + (c >> 4);
+    const bool v = pr < npairs && it < a.nitems;
+    int fr = 0, sa = 0;
+    if (v) {
+      fr = (int)(it / a.nactive);
+      sa = a.active[it - (long)fr * a.nactive];
     }
-    const int jj = c & 15;
-    float er[8], ei[8];
-    {
-      const int sy = s1 / a.nsubx, sx = s1 - (s1 / a.nsubx) * a.nsubx;
-      const float *ph = a.phs + (long)frame1 * a.phs_stride + (long)(sy * n) * a.npup + sx * n;
-      const float *pu = a.pup + (long)s1 * n * n;
+    const int sy = sa / a.nsubx, sx = sa - (sa / a.nsubx) * a.nsubx;
+    const float *ph = a.phs + (long)fr * a.phs_stride + (long)(sy * n) * a.npup + sx * n;
+    const float *pu = a.pup + (long)sa * n * n;
 #pragma unroll
-      for (int t = 0; t < 8; ++t) {
-        const int i = 8 * h + t;
-        float vr = 0.f, vi = 0.f;
-        if (v1 && i < n && jj < n) {
-          const float th = ph[(long)i * a.npup + jj] - a.tilt[i * n + jj];
-          float sn, cs;
-          sincosf(th, &sn, &cs);
-          const float p = pu[i * n + jj];
-          vr = p * cs;
-          vi = p * sn;
-        }
-        er[t] = vr;
-        ei[t] = vi;
-      }
+    for (int t = 0; t < 8; ++t) {
+      const int i = 8 * h + t;
+      const bool in = v && i < n && jj < n;
+      ph8[t] = in ? ph[(long)i * a.npup + jj] : 0.f;
+      pu8[t] = in ? pu[i * n + jj] : 0.f;
+    }
+  };
+  float cph[8], cpu[8];
+  load_pair(wave_id, cph, cpu);
+  for (long pr = wave_id; pr < npairs; pr += nwaves) {
+    float nph[8], npu[8];
+    load_pair(pr + nwaves, nph, npu);   // next pair's loads overlap this pair's work
+    float er[8], ei[8];
+#pragma unroll
+    for (int t = 0; t < 8; ++t) {
+      const int i = 8 * h + t;
+      const float th = cph[t] - a.tiltstep * (float)(i + jj + 1 - n);   // float ops as the reference
+      float sn, cs;
+      dft_sincos<FAST>(th, &sn, &cs);
+      er[t] = cpu[t] * cs;
+      ei[t] = cpu[t] * sn;
+    }
+#pragma unroll
+    for (int t = 0; t < 8; ++t) {
+      cph[t] = nph[t];
+      cpu[t] = npu[t];
     }
     COp<T8> A1;
     A1.set(er, ei);
     f16v zr = {}, zi = {};
-    cgemm(A1, B1, zr, zi);   // Z^T stacked: rows (s, j), columns k
+    cgemm_vc(A1, B1, zr, zi);   // Z^T stacked: rows (s, j), columns k
 
 #pragma unroll
     for (int sub = 0; sub < 2; ++sub) {
@@ -181,7 +236,7 @@ __global__ __launch_bounds__(256) void dft16_cog_kernel(FusedArgs a) {
       COp<T8> B2;
       B2.set(br, bi);
       f16v xr = {}, xi = {};
-      cgemm(A2, B2, xr, xi);   // X^T: rows l, columns k (lane)
+      cgemm_cv(A2, B2, xr, xi);   // X^T: rows l, columns k (lane)
       // ---- |X|^2 binned 2x2: in-lane along l, lane pairs along k ----
       // register r holds l = 8(r/4) + 4h + r%4; pairs (r, r+1) for even r share a detector column
       float pix[8];
@@ -197,51 +252,70 @@ __global__ __launch_bounds__(256) void dft16_cog_kernel(FusedArgs a) {
       const int I = (kF - 2 - c) >> 1;
       const int M2 = kM * kM;
       const long pix0 = (long)s * M2;
-      const long noff = (long)frame * a.noise_stride + pix0;
       const float scale = a.scale[s];
       const float skyadd = a.skyadd[s];
-      const bool pois = a.poisson_on && a.poisson_on[s];
-      const bool rn = a.readnoise != 0.f && a.zread;
-      float mx = -INFINITY;
-#pragma unroll
-      for (int q = 0; q < 8; ++q) {
-        // l pair starts at l0 = 8(q/2) + 4h + 2(q%2): detector column J = (F-2-l0)/2
-        const int J = (kF - 2 - (8 * (q >> 1) + 4 * h + 2 * (q & 1))) >> 1;
-        float v = pix[q] * scale;
-        if (owner) {
-          const long p = I * kM + J;
-          if (skyadd > 0.f) v += skyadd;
-          if (a.skyarr) v += a.skyarr[pix0 + p];
-          if (pois) v = poisson_from_normal(v, a.zpois[noff + p]);
-          if (a.threshType >= 0 && a.threshType <= 2) v += rn ? fmaf(a.readnoise, a.zread[noff + p], a.readbg) : a.readbg;
-          mx = fmaxf(mx, v);
-        }
-        pix[q] = v;
-      }
-      float thr = a.floor;
-      if (a.threshType == 2) {
-        mx = wave_max(mx);
-        if (a.floor > 0.f && a.floor <= 1.f) thr = a.floor * fmaxf(mx, 0.f);
-      }
       const int st0 = (kM - a.ncen) / 2, en0 = kM - st0;
       float sw = 0.f, sxw = 0.f, syw = 0.f;
-      float *bout = a.bimg ? a.bimg + (long)frame * a.bimg_stride + pix0 : nullptr;
-      const float *wt = a.centWeight ? a.centWeight + (a.centWeightDim == 4 ? pix0 : 0) : nullptr;
+      if (!a.slow_detector) {
+        const float add = (skyadd > 0.f ? skyadd : 0.f) + (a.threshType >= 0 && a.threshType <= 2 ? a.readbg : 0.f);
+        const float thr = a.floor;
 #pragma unroll
-      for (int q = 0; q < 8; ++q) {
-        const int J = (kF - 2 - (8 * (q >> 1) + 4 * h + 2 * (q & 1))) >> 1;
-        float v = pix[q];
-        if (a.threshType == 1) v = v < thr ? 0.f : v;
-        else if (a.threshType == 0 || a.threshType == 2) v = v < thr ? 0.f : v - thr;
-        if (a.pxlPower == 2.f) v = v * v;
-        else if (a.pxlPower != 1.f && a.pxlPower != 0.f) v = powf(v, a.pxlPower);
-        if (owner) {
-          if (bout) bout[I * kM + J] = v;
-          if (I >= st0 && I < en0 && J >= st0 && J < en0) {
-            const float tt = wt ? v * wt[I * kM + J] : v;
-            sw += tt;
-            sxw += tt * J;
-            syw += tt * I;
+        for (int q = 0; q < 8; ++q) {
+          // l pair starts at l0 = 8(q/2) + 4h + 2(q%2): detector column J = (F-2-l0)/2
+          const int J = (kF - 2 - (8 * (q >> 1) + 4 * h + 2 * (q & 1))) >> 1;
+          float v = pix[q] * scale + add;
+          if (a.threshType == 1) v = v < thr ? 0.f : v;
+          else if (a.threshType == 0) v = v < thr ? 0.f : v - thr;
+          if (owner && I >= st0 && I < en0 && J >= st0 && J < en0) {
+            sw += v;
+            sxw += v * J;
+            syw += v * I;
+          }
+        }
+      } else {
+#pragma unroll
+        for (int q = 0; q < 8; ++q) px[q] = pix[q];
+        const long noff = (long)frame * a.noise_stride + pix0;
+        const bool pois = a.poisson_on && a.poisson_on[s];
+        const bool rn = a.readnoise != 0.f && a.zread;
+        float mx = -INFINITY;
+#pragma unroll 1
+        for (int q = 0; q < 8; ++q) {
+          const int J = (kF - 2 - (8 * (q >> 1) + 4 * h + 2 * (q & 1))) >> 1;
+          float v = px[q] * scale;
+          if (owner) {
+            const long p = I * kM + J;
+            if (skyadd > 0.f) v += skyadd;
+            if (a.skyarr) v += a.skyarr[pix0 + p];
+            if (pois) v = poisson_from_normal(v, a.zpois[noff + p]);
+            if (a.threshType >= 0 && a.threshType <= 2) v += rn ? fmaf(a.readnoise, a.zread[noff + p], a.readbg) : a.readbg;
+            mx = fmaxf(mx, v);
+          }
+          px[q] = v;
+        }
+        float thr = a.floor;
+        if (a.threshType == 2) {
+          mx = wave_max(mx);
+          if (a.floor > 0.f && a.floor <= 1.f) thr = a.floor * fmaxf(mx, 0.f);
+        }
+        float *bout = a.bimg ? a.bimg + (long)frame * a.bimg_stride + pix0 : nullptr;
+        const float *wt = a.centWeight ? a.centWeight + (a.centWeightDim == 4 ? pix0 : 0) : nullptr;
+#pragma unroll 1
+        for (int q = 0; q < 8; ++q) {
+          const int J = (kF - 2 - (8 * (q >> 1) + 4 * h + 2 * (q & 1))) >> 1;
+          float v = px[q];
+          if (a.threshType == 1) v = v < thr ? 0.f : v;
+          else if (a.threshType == 0 || a.threshType == 2) v = v < thr ? 0.f : v - thr;
+          if (a.pxlPower == 2.f) v = v * v;
+          else if (a.pxlPower != 1.f && a.pxlPower != 0.f) v = powf(v, a.pxlPower);
+          if (owner) {
+            if (bout) bout[I * kM + J] = v;
+            if (I >= st0 && I < en0 && J >= st0 && J < en0) {
+              const float tt = wt ? v * wt[I * kM + J] : v;
+              sw += tt;
+              sxw += tt * J;
+              syw += tt * I;
+            }
           }
         }
       }
@@ -274,10 +348,13 @@ hipError_t launch_dft(const FusedArgs &a, int bf16, hipStream_t st) {
   long blocks = (npairs + 3) / 4;
   const long cap = 256L * 8 * 4;   // enough resident waves to fill the chip, then grid-stride
   if (blocks > cap) blocks = cap;
-  if (bf16)
-    hipLaunchKernelGGL(dft16_cog_kernel<b8>, dim3((unsigned)blocks), dim3(256), 0, st, a);
-  else
-    hipLaunchKernelGGL(dft16_cog_kernel<h8>, dim3((unsigned)blocks), dim3(256), 0, st, a);
+  if (bf16) {
+    if (a.fast_trig) hipLaunchKernelGGL((dft16_cog_kernel<b8, true>), dim3((unsigned)blocks), dim3(256), 0, st, a);
+    else hipLaunchKernelGGL((dft16_cog_kernel<b8, false>), dim3((unsigned)blocks), dim3(256), 0, st, a);
+  } else {
+    if (a.fast_trig) hipLaunchKernelGGL((dft16_cog_kernel<h8, true>), dim3((unsigned)blocks), dim3(256), 0, st, a);
+    else hipLaunchKernelGGL((dft16_cog_kernel<h8, false>), dim3((unsigned)blocks), dim3(256), 0, st, a);
+  }
   return hipGetLastError();
 }
 

@@ -142,6 +142,24 @@ struct FusedArgs {
   long bimg_stride;
 };
 
+// Multi-WFS persistent step (batch_kernel.h): one descriptor per WFS in a shape's launch.
+struct BatchDesc {
+  FusedArgs args;
+  long group0;           // first queue entry of this WFS (groups of 64/NP work items)
+};
+
+struct BatchArgs {
+  const BatchDesc *desc; // ascending group0, desc[0].group0 == 0
+  int ndesc;
+  long total_groups;
+  unsigned long long *counters;   // [0] next queue entry, [1] waves finished; zero between launches
+};
+
+bool batch_supported(int n_pad, int F, int nimg);
+int batch_resident_wgs(int n_pad, int F, int nimg, int waves_per_wg, int fast);   // per CU
+hipError_t launch_batch(const BatchArgs &a, int n_pad, int F, int nimg, int waves_per_wg, int fast,
+                        long workgroups, hipStream_t st);
+
 hipError_t launch_form(const FormArgs &a, int n_pad, int F, hipStream_t st);
 hipError_t launch_backend(const BackArgs &a, hipStream_t st);
 hipError_t launch_inactive(const InactiveArgs &a, hipStream_t st);

@@ -1,0 +1,269 @@
+// fused_item.h - device code of the fused FP32 path shared by the single-WFS kernel
+// (cent_fused.hip) and the multi-WFS persistent kernel (cent_batch.hip): phase -> |FFT|^2 ->
+// binned detector image -> noise -> threshold -> CoG for one work item per NP-lane group.
+//
+// Layout (wave64): a wave owns SPW = 64/NP subapertures, lane (s, j) owns phase column j of
+// subaperture s. Pass 1 transforms columns along i in registers. The transpose to rows goes
+// through the wave's private LDS region in R = F/NP phases of NP rows each (so only NP x NP
+// complex values per subaperture are staged at a time), and pass 2 transforms row k = m*R + r
+// along j. |X|^2 is binned straight from registers: with clip == F the 180-degree rotation maps
+// row k to detector row (F-1-k)/B and column l to (F-1-l)/B, all compile-time indices.
+#pragma once
+#include "cent_internal.h"
+#include "fft_reg.h"
+#include "noise.h"
+
+namespace dasp {
+
+template <int NP, int F, int B>
+struct FusedShape {
+  static constexpr int SPW = 64 / NP;
+  static constexpr int R = F / NP;
+  static constexpr int NIMG = F / B;
+  static constexpr int RB = R >= B ? R / B : 1;   // detector rows held per lane
+  static constexpr int G = B >= R ? B / R : 1;    // lanes that share one detector row
+  static constexpr int TS = NP + 1;               // staged row stride (complex)
+  static constexpr int WAVES = 4;
+  static_assert(R * NP == F && NIMG * B == F, "shape");
+  static_assert((R % B == 0) || (B % R == 0), "binning must align with rows per lane");
+};
+
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template <int W>
+__device__ __forceinline__ float group_sum(float v) {
+  static_for<0, 6>([&](auto b) {
+    constexpr int m = 1 << decltype(b)::value;
+    if constexpr (m < W) v += __shfl_xor(v, m);
+  });
+  return v;
+}
+
+template <int W>
+__device__ __forceinline__ float group_max(float v) {
+  static_for<0, 6>([&](auto b) {
+    constexpr int m = 1 << decltype(b)::value;
+    if constexpr (m < W) v = fmaxf(v, __shfl_xor(v, m));
+  });
+  return v;
+}
+
+// exp(i th) for the phase argument. Accurate path: sincosf. Fast path: Cody-Waite reduction by
+// 2 pi (two-constant split) to |rem| <= pi, then the hardware sine/cosine of rem/(2 pi) revolutions.
+template <bool FAST>
+__device__ __forceinline__ void phase_sincos(float th, float *sn, float *cs) {
+  if constexpr (FAST) {
+    const float k = rintf(th * 0.15915494309189535f);
+    float rem = fmaf(-k, 6.28318548202514648f, th);
+    rem = fmaf(-k, -1.7484556000744883e-07f, rem);
+    const float r = rem * 0.15915494309189535f;
+    *sn = __builtin_amdgcn_sinf(r);
+    *cs = __builtin_amdgcn_cosf(r);
+  } else {
+    sincosf(th, sn, cs);
+  }
+}
+
+template <int NP>
+struct PhaseCol {
+  float ph[NP], pu[NP];
+};
+
+// Load phase column j (rows i < n) and the pupil of one work item; zeros where out of range.
+template <int NP>
+__device__ __forceinline__ void load_item(const FusedArgs &a, long item, int j, PhaseCol<NP> &c) {
+  const bool valid = item < a.nitems;
+  int frame = 0, s = 0;
+  if (valid) {
+    frame = (int)(item / a.nactive);
+    s = a.active[item - (long)frame * a.nactive];
+  }
+  const int n = a.n;
+  const int sy = s / a.nsubx, sx = s - (s / a.nsubx) * a.nsubx;
+  const float *ph = a.phs + (long)frame * a.phs_stride + (long)(sy * n) * a.npup + sx * n;
+  const float *pu = a.pup + (long)s * n * n;
+  static_for<0, NP>([&](auto ii) {
+    constexpr int i = decltype(ii)::value;
+    const bool in = valid && i < n && j < n;
+    c.ph[i] = in ? ph[(long)i * a.npup + j] : 0.f;
+    c.pu[i] = in ? pu[i * n + j] : 0.f;
+  });
+}
+
+// One work item (frame, subaperture) per NP-lane group: image formation, detector, CoG.
+template <int NP, int F, int B, bool FAST>
+__device__ __forceinline__ void fused_item(const FusedArgs &a, long item, int j, int w, int sl,
+                                           const PhaseCol<NP> &col, cf *st) {
+  using S = FusedShape<NP, F, B>;
+  constexpr int R = S::R, NIMG = S::NIMG, RB = S::RB, G = S::G, TS = S::TS;
+  const bool valid = item < a.nitems;
+  int frame = 0, s = 0;
+  if (valid) {
+    frame = (int)(item / a.nactive);
+    s = a.active[item - (long)frame * a.nactive];
+  }
+  const int n = a.n;
+  // ---- pass 1 operand: E[i][j] = pup * exp(i (phi - t)) ----
+  cf x[NP];
+  static_for<0, NP>([&](auto ii) {
+    constexpr int i = decltype(ii)::value;
+    const float th = col.ph[i] - a.tiltstep * (float)(i + j + 1 - n);   // float ops as the reference
+    float sn, cs;
+    phase_sincos<FAST>(th, &sn, &cs);
+    x[i] = {col.pu[i] * cs, col.pu[i] * sn};
+  });
+
+  // ---- transpose in R phases + pass 2 + binning ----
+  // Phase r stages rows k = q*R + r (q < NP): pass-1 output Z[q*R + r][j] = FFT_NP(x[i] w_F^{ir})[q],
+  // so each phase needs one NP-point transform per lane and nothing else stays live.
+  float acc[RB][NIMG];
+  static_for<0, RB>([&](auto ib) {
+    static_for<0, NIMG>([&](auto jb) { acc[decltype(ib)::value][decltype(jb)::value] = 0.f; });
+  });
+  static_for<0, R>([&](auto rr) {
+    constexpr int r = decltype(rr)::value;
+    {
+      cf y[NP];
+      static_for<0, NP>([&](auto ii) {
+        constexpr int i = decltype(ii)::value;
+        y[i] = twiddle<i * r, F>(x[i]);
+      });
+      fft_fwd<NP>(y);
+      static_for<0, NP>([&](auto qq) { st[decltype(qq)::value * TS + j] = y[decltype(qq)::value]; });
+    }
+    wave_lds_sync();
+    cf row[NP];
+    static_for<0, NP>([&](auto jj) { row[decltype(jj)::value] = st[j * TS + decltype(jj)::value]; });
+    wave_lds_sync();
+    constexpr int ib = R >= B ? r / B : 0;
+    // row k = j*R + r along its columns: X[R*q2 + r2] = FFT_NP(row[jj] w_F^{jj r2})[q2]
+    static_for<0, R>([&](auto rr2) {
+      constexpr int r2 = decltype(rr2)::value;
+      cf y2[NP];
+      static_for<0, NP>([&](auto jj) {
+        constexpr int jc = decltype(jj)::value;
+        y2[jc] = twiddle<jc * r2, F>(row[jc]);
+      });
+      fft_fwd<NP>(y2);
+      static_for<0, NP>([&](auto qq) {
+        constexpr int l = R * decltype(qq)::value + r2;
+        constexpr int jb = (F - 1 - l) / B;
+        acc[ib][jb] += cabs2(y2[decltype(qq)::value]);
+      });
+    });
+  });
+  if constexpr (G > 1) {
+    static_for<0, RB>([&](auto ib) {
+      static_for<0, NIMG>([&](auto jb) {
+        acc[decltype(ib)::value][decltype(jb)::value] = group_sum<G>(acc[decltype(ib)::value][decltype(jb)::value]);
+      });
+    });
+  }
+  const bool owner = valid && (G == 1 || (j % G) == 0);
+  // detector row of acc[ib]: lane (row group j) holds rows k = j*R .. j*R+R-1
+  auto det_row = [&](int ib) { return R >= B ? (F - j * R) / B - 1 - ib : (F - 1 - j * R) / B; };
+
+  // ---- detector: normalise, sky, noise, threshold, power, CoG window sums ----
+  const int M2 = NIMG * NIMG;
+  const long pix0 = (long)s * M2;
+  const float scale = valid ? a.scale[s] : 1.f;
+  const float skyadd = valid ? a.skyadd[s] : 0.f;
+  const int st0 = (NIMG - a.ncen) / 2, en0 = NIMG - st0;
+  float sw = 0.f, sxw = 0.f, syw = 0.f;
+  if (!a.slow_detector) {
+    // no noise, sky array, relative floor, power, weights or image output: all inline
+    const float add = (skyadd > 0.f ? skyadd : 0.f) + (a.threshType >= 0 && a.threshType <= 2 ? a.readbg : 0.f);
+    const float thr = a.floor;
+    static_for<0, RB>([&](auto ibb) {
+      constexpr int ib = decltype(ibb)::value;
+      const int I = det_row(ib);
+      static_for<0, NIMG>([&](auto jbb) {
+        constexpr int jb = decltype(jbb)::value;
+        float v = fmaf(acc[ib][jb], scale, 0.f) + add;
+        if (a.threshType == 1) v = v < thr ? 0.f : v;
+        else if (a.threshType == 0) v = v < thr ? 0.f : v - thr;
+        if (owner && I >= st0 && I < en0 && jb >= st0 && jb < en0) {
+          sw += v;
+          sxw += v * jb;
+          syw += v * I;
+        }
+      });
+    });
+  } else {
+    // general detector: stage the lane's pixels in its LDS slot and walk them with a
+    // runtime loop, so that the rarely used per-pixel code is emitted once
+    float *px = reinterpret_cast<float *>(st) + j * (RB * NIMG);
+    static_for<0, RB>([&](auto ib) {
+      static_for<0, NIMG>([&](auto jb) {
+        px[decltype(ib)::value * NIMG + decltype(jb)::value] = acc[decltype(ib)::value][decltype(jb)::value];
+      });
+    });
+    const long noff = (long)frame * a.noise_stride + pix0;
+    const bool pois = valid && a.poisson_on && a.poisson_on[s];
+    const bool rn = a.readnoise != 0.f && a.zread;
+    float mx = -INFINITY;
+#pragma unroll 1
+    for (int q = 0; q < RB * NIMG; ++q) {
+      const int I = det_row(q / NIMG), jb = q % NIMG;
+      float v = px[q] * scale;   // scale is 1 where the theoretical total is not positive
+      if (owner) {
+        const long p = I * NIMG + jb;
+        if (skyadd > 0.f) v += skyadd;
+        if (a.skyarr) v += a.skyarr[pix0 + p];
+        if (pois) v = poisson_from_normal(v, a.zpois[noff + p]);
+        if (a.threshType >= 0 && a.threshType <= 2) v += rn ? fmaf(a.readnoise, a.zread[noff + p], a.readbg) : a.readbg;
+        mx = fmaxf(mx, v);
+      }
+      px[q] = v;
+    }
+    float thr = a.floor;
+    if (a.threshType == 2) {
+      mx = group_max<NP>(mx);
+      if (a.floor > 0.f && a.floor <= 1.f) thr = a.floor * fmaxf(mx, 0.f);
+    }
+    float *bout = a.bimg ? a.bimg + (long)frame * a.bimg_stride + pix0 : nullptr;
+    const float *wt = a.centWeight ? a.centWeight + (a.centWeightDim == 4 ? pix0 : 0) : nullptr;
+#pragma unroll 1
+    for (int q = 0; q < RB * NIMG; ++q) {
+      const int I = det_row(q / NIMG), jb = q % NIMG;
+      float v = px[q];
+      if (a.threshType == 1) v = v < thr ? 0.f : v;
+      else if (a.threshType == 0 || a.threshType == 2) v = v < thr ? 0.f : v - thr;
+      if (a.pxlPower == 2.f) v = v * v;
+      else if (a.pxlPower != 1.f && a.pxlPower != 0.f) v = powf(v, a.pxlPower);
+      if (owner) {
+        if (bout) bout[I * NIMG + jb] = v;
+        if (I >= st0 && I < en0 && jb >= st0 && jb < en0) {
+          const float t = wt ? v * wt[I * NIMG + jb] : v;
+          sw += t;
+          sxw += t * jb;
+          syw += t * I;
+        }
+      }
+    }
+    wave_lds_sync();   // the slot is the next item's transpose buffer
+  }
+  sw = group_sum<NP>(sw);
+  sxw = group_sum<NP>(sxw);
+  syw = group_sum<NP>(syw);
+  if (valid && j == 0) {
+    float cx = 0.f, cy = 0.f;
+    if (sw > 0.f) {
+      cx = sxw / sw - (NIMG - 1) / 2.f;
+      cy = syw / sw - (NIMG - 1) / 2.f;
+    }
+    if (a.refCents) {
+      cx -= a.refCents[s * 2];
+      cy -= a.refCents[s * 2 + 1];
+    }
+    float *co = a.cents + (long)frame * a.cents_stride + s * 2;
+    co[0] = cx;
+    co[1] = cy;
+  }
+}
+
+}  // namespace dasp

@@ -156,8 +156,9 @@ int dasp_cent_testcorrelation(const float *img, float *corr, int n);
 /* Select the compute form (DASP_CENT_FORM_*). */
 int dasp_cent_set_form(dasp_cent_t *h, int form);
 
-/* Phase factor exp(i(phi - t)) in the fused kernels: 0 = sincosf (default), 1 = two-constant
- * Cody-Waite reduction by 2 pi and the hardware sine/cosine (v_sin_f32/v_cos_f32). */
+/* Phase factor exp(i(phi - t)) in the fused kernels: 1 (default) = two-constant Cody-Waite
+ * reduction by 2 pi and the hardware sine/cosine (v_sin_f32/v_cos_f32; max abs error 3.8e-7 on
+ * |phi - t| <= 400 rad), 0 = sincosf (max abs error 7e-8). The general kernels use sincosf. */
 int dasp_cent_set_fast_trig(dasp_cent_t *h, int enable);
 
 /* Device-resident step over nframes phase frames (science/wfscent.py:630-702 keeps phase,

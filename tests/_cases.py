@@ -45,7 +45,7 @@ class Case:
         if form is not None:
             from dasp_amd.cmod import cent
             cent.setForm(self.cc.centstruct, form % 10)
-            cent.setFastTrig(self.cc.centstruct, form >= 10)
+            cent.setFastTrig(self.cc.centstruct, form >= 10)   # 0..3: accurate sincosf, 10..13: fast
         gf = 0 if gaussianFitVals is None else 1
         sigarr = sig if isinstance(sig, np.ndarray) else None
         skyarr = skybrightness if isinstance(skybrightness, np.ndarray) else None
