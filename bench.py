@@ -70,7 +70,9 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--form", default="fft32", choices=["fft32", "bf16", "fp16"])
-    ap.add_argument("--workload", default="c3", choices=["c1", "c2", "c3", "c5"])
+    ap.add_argument("--workload", default="c3", choices=["c1", "c2", "c3", "c5", "c4"])
+    ap.add_argument("--batch", action="store_true",
+                    help="run the single WFS through the multi-WFS step (persistent queue kernel + graph)")
     ap.add_argument("--frames", type=int, default=0, help="frames per step (default: the workload's)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--accurate-trig", action="store_true", help="sincosf instead of the reduced hardware sin/cos")
@@ -90,6 +92,8 @@ def main():
 
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
+    if args.workload == "c4":
+        return bench_multi(args, dev)
     w = {"c1": W.C1, "c2": W.C2, "c3": W.C3, "c5": W.C5}[args.workload]
     frames = args.frames or w.nframes
     S = w.nsubx * w.nsubx
@@ -125,8 +129,20 @@ def main():
     if w.correlation:
         d_corr = torch.zeros((frames, S, csz, csz), dtype=torch.float32, device=dev)
     stream = torch.cuda.current_stream()
+    wb = None
+    if args.batch:
+        from dasp_amd.wfsbatch import WFSBatch
+        wb = WFSBatch([cc], frames)
+        wb.phs[0].copy_(phs)
+        d_cents = wb.cents[0].view(frames, S, 2)
+        if noise:
+            raise SystemExit("--batch: the noise arrays of the multi-WFS step are filled by WFSBatch.draw_noise")
+        wb.tune(reps=3)
 
     def step():
+        if wb is not None:
+            wb.run()
+            return
         cent.runDevice(h, phs.data_ptr(), npup * npup, frames, d_cents.data_ptr(), 0,
                        d_corr.data_ptr() if d_corr is not None else 0,
                        zp.data_ptr() if zp is not None else 0, zr.data_ptr() if zr is not None else 0,
@@ -177,13 +193,27 @@ def main():
                    "npup": w.npup, "frames_per_step": frames, "active_subaps": nact, "form": args.form,
                    "noise": noise, "images_returned": False, "fast_trig": not args.accurate_trig, "wall_s": wall},
     }
+    if wb is not None:
+        launches, _ = wb.info()
+        kms, klaunch = launches[0]["tuned_ms"], 1
+        result["config"]["path"] = "multi-WFS step (1 WFS): %s" % launches[0]
     if klaunch > 0:
         kavg = kms / klaunch / 1e3
         abytes = algorithmic_bytes(w, nact, frames, False, noise)
         ach = abytes / kavg / 1e9
-        result["roofline"] = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                              "frac": ach / HBM_PEAK_GBS, "traffic": None,
-                              "kernel_avg_ms": kavg * 1e3, "algorithmic_bytes": abytes}
+        if args.form == "fft32":
+            result["roofline"] = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                  "frac": ach / HBM_PEAK_GBS, "traffic": None,
+                                  "kernel_avg_ms": kavg * 1e3, "algorithmic_bytes": abytes}
+        else:
+            # DFT-GEMM form: 18 v_mfma_f32_32x32x16 per subaperture image (hi/lo, 3 products)
+            flops = 18 * 2 * 32 * 32 * 16 * nact * frames
+            tf = flops / kavg / 1e12
+            result["roofline"] = {"bound": "mfma", "achieved": tf, "peak": MFMA_PEAK_TFLOPS[args.form],
+                                  "unit": "TFLOP/s", "frac": tf / MFMA_PEAK_TFLOPS[args.form], "traffic": None,
+                                  "kernel_avg_ms": kavg * 1e3, "mfma_flops": flops,
+                                  "useful_flops": 8 * (w.nfft * w.n ** 2 + w.nfft ** 2 * w.n) * nact * frames,
+                                  "hbm_achieved_gbs": ach, "algorithmic_bytes": abytes}
     if not args.no_cpu:
         # CPU leg: the FP64 oracle on frame 0 of the same workload; its slopes also give the
         # engine's error against FP64.
@@ -195,8 +225,81 @@ def main():
             d = (d_cents[0].cpu().numpy() - oc)[act]
             result["rms_err_vs_fp64_px"] = float(np.sqrt((d ** 2).mean()))
             result["max_err_vs_fp64_px"] = float(np.abs(d).max())
+    if wb is not None:
+        wb.free()
     cc.free()
     print(json.dumps(result), flush=True)
+
+
+def bench_multi(args, dev):
+    """Multi-WFS step: 6 LGS 80x80 (n16/nfft32/nimg16, per-subaperture spot kernels), 3 NGS 8x8
+    (n32/nfft64/nimg8) and 1 truth 40x40 (n16/nfft32/nimg16), Poisson + read noise, independent
+    seeded phase; every frame of every WFS in one step graph (persistent launches per shape)."""
+    import torch
+    from dasp_amd import workloads as W, synth
+    from dasp_amd.util import centcmod
+    from dasp_amd.wfsbatch import WFSBatch
+    frames = args.frames or 16
+    geo = [(W.C4_LGS, k) for k in range(6)] + [(W.C4_NGS, 6 + k) for k in range(3)] + [(W.C4_TRUTH, 9)]
+    ccs, hosts, spot_cache = [], [], {}
+    for w, k in geo:
+        pupil = w.pupil()
+        host_phs = np.zeros((w.npup, w.npup), np.float32)
+        cents_h = np.zeros((w.nsubx, w.nsubx, 2), np.float32)
+        bimg_h = np.zeros((w.nsubx, w.nsubx, w.nimg, w.nimg), np.float32)
+        if w.name not in spot_cache:
+            spot_cache[w.name] = w.spots()
+        spots = spot_cache[w.name]
+        cc = centcmod.centcmod(1, w.nsubx, w.cen, w.nfft, w.clip, w.nimg, w.n, w.readnoise, 0., w.addPoisson,
+                               w.noiseFloor, float(w.sig), 0., 0, 1., 1, 0, 1 + k, host_phs, pupil, spots, cents_h,
+                               bimg_h, 0.5, 0, None, 0, 0., None, None, w.threshType, 0, 0, 1, 0, None)
+        cent_mod().setFastTrig(cc.centstruct, not args.accurate_trig)
+        ccs.append(cc)
+        hosts.append((host_phs, cents_h, bimg_h))
+    wb = WFSBatch(ccs, frames)
+    gen = torch.Generator(device=dev)
+    for k, (w, seed) in enumerate(geo):
+        wb.phs[k].copy_(synth.phase_screens_torch(frames, w.npup, w.D, w.r0, w.L0, 10 + seed, dev,
+                                                  wavelength_scale=500. / 640.))
+        gen.manual_seed(1000 + seed)
+        for z in (wb.zpois[k], wb.zread[k]):
+            if z is not None:
+                z.copy_(torch.randn(z.shape, generator=gen, device=dev))
+    torch.cuda.synchronize()
+    launches, others = wb.tune(reps=3)
+    stream = torch.cuda.current_stream()
+    for _ in range(args.warmup):
+        wb.run()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    e0.record(stream)
+    for _ in range(args.steps):
+        wb.run()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms = e0.elapsed_time(e1) / args.steps
+    per_step = wb.subap_images_per_step()
+    result = {
+        "metric": "subap images+centroids/s, multi-WFS step (6 LGS + 3 NGS + 1 truth)",
+        "value": per_step / (ms / 1e3), "unit": "subap-images/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "c4_multi_wfs", "frames_per_step": frames, "subap_images_per_step": per_step,
+                   "launches": launches, "wfs_outside_persistent_launches": others,
+                   "fast_trig": not args.accurate_trig, "wall_s": wall},
+    }
+    wb.free()
+    for cc in ccs:
+        cc.free()
+    print(json.dumps(result), flush=True)
+
+
+def cent_mod():
+    from dasp_amd.cmod import cent
+    return cent
 
 
 if __name__ == "__main__":

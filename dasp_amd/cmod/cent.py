@@ -46,6 +46,12 @@ class _Params(ctypes.Structure):
     ]
 
 
+class _BatchItem(ctypes.Structure):
+    _fields_ = [("handle", ctypes.c_void_p), ("phs", ctypes.c_void_p), ("phs_stride", ctypes.c_long),
+                ("nframes", ctypes.c_int), ("cents", ctypes.c_void_p), ("bimg", ctypes.c_void_p),
+                ("corrimg", ctypes.c_void_p), ("zpois", ctypes.c_void_p), ("zread", ctypes.c_void_p)]
+
+
 _lib = None
 
 
@@ -83,6 +89,20 @@ def library():
         lib.dasp_cent_set_timing.argtypes = [vp, ip]
         lib.dasp_cent_set_fast_trig.argtypes = [vp, ip]
         lib.dasp_cent_kernel_time.argtypes = [vp, ip, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_long)]
+        lib.dasp_cent_batch_create.argtypes = [ctypes.POINTER(_BatchItem), ip, ctypes.POINTER(vp)]
+        lib.dasp_cent_batch_tune.argtypes = [vp, vp, ip]
+        lib.dasp_cent_batch_run.argtypes = [vp, vp]
+        lib.dasp_cent_batch_launches.argtypes = [vp]
+        lib.dasp_cent_batch_launch_info.argtypes = [vp, ip, ctypes.POINTER(ip), ctypes.POINTER(ip),
+                                                    ctypes.POINTER(ctypes.c_long), ctypes.POINTER(ip),
+                                                    ctypes.POINTER(ctypes.c_long), ctypes.POINTER(ip),
+                                                    ctypes.POINTER(ctypes.c_float)]
+        lib.dasp_cent_batch_others.argtypes = [vp]
+        lib.dasp_cent_batch_free.argtypes = [vp]
+        for name in ("dasp_cent_batch_create", "dasp_cent_batch_tune", "dasp_cent_batch_run",
+                     "dasp_cent_batch_launches", "dasp_cent_batch_launch_info", "dasp_cent_batch_others",
+                     "dasp_cent_batch_free"):
+            getattr(lib, name).restype = ctypes.c_int
         for name in ("dasp_cent_initialise", "dasp_cent_run", "dasp_cent_run_slope", "dasp_cent_update_int",
                      "dasp_cent_update_float", "dasp_cent_update_array", "dasp_cent_update_caldata",
                      "dasp_cent_free", "dasp_cent_testcorrelation", "dasp_cent_set_form",
@@ -329,3 +349,58 @@ def kernelTime(handle, reset=False):
     ms, n = ctypes.c_double(), ctypes.c_long()
     _check(library().dasp_cent_kernel_time(_ptr(handle), int(bool(reset)), ctypes.byref(ms), ctypes.byref(n)))
     return ms.value, n.value
+
+
+# ---- multi-WFS step ------------------------------------------------------------------------
+def batchCreate(items):
+    """Build a multi-WFS step. items: sequence of dicts with keys handle, phs, phs_stride, nframes,
+    cents and optionally bimg, corrimg, zpois, zread (device addresses). Returns a batch handle."""
+    arr = (_BatchItem * len(items))()
+    for k, it in enumerate(items):
+        arr[k].handle = _ptr(it["handle"])
+        arr[k].phs = it["phs"]
+        arr[k].phs_stride = int(it["phs_stride"])
+        arr[k].nframes = int(it["nframes"])
+        arr[k].cents = it["cents"]
+        for key in ("bimg", "corrimg", "zpois", "zread"):
+            setattr(arr[k], key, it.get(key) or None)
+    out = ctypes.c_void_p()
+    _check(library().dasp_cent_batch_create(arr, len(items), ctypes.byref(out)))
+    return out.value
+
+
+def batchTune(batch, stream=0, reps=5):
+    """Autotune waves per workgroup and workgroup count of every persistent launch (synchronous)."""
+    _check(library().dasp_cent_batch_tune(batch, stream or None, int(reps)))
+
+
+def batchRun(batch, stream=0):
+    """One step: a single graph launch on stream (asynchronous)."""
+    _check(library().dasp_cent_batch_run(batch, stream or None))
+
+
+def batchInfo(batch):
+    """List of dicts describing the persistent launches, and the number of WFS outside them."""
+    lib = library()
+    n = lib.dasp_cent_batch_launches(batch)
+    if n < 0:
+        _check(n)
+    out = []
+    for i in range(n):
+        shape = (ctypes.c_int * 4)()
+        nwfs, waves, chunk = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        groups, wgs = ctypes.c_long(), ctypes.c_long()
+        ms = ctypes.c_float()
+        _check(lib.dasp_cent_batch_launch_info(batch, i, shape, ctypes.byref(nwfs), ctypes.byref(groups),
+                                               ctypes.byref(waves), ctypes.byref(wgs), ctypes.byref(chunk),
+                                               ctypes.byref(ms)))
+        out.append(dict(np=shape[0], fftsize=shape[1], nimg=shape[2], spot=bool(shape[3]), nwfs=nwfs.value, groups=groups.value,
+                        waves_per_wg=waves.value, workgroups=wgs.value, chunk=chunk.value, tuned_ms=ms.value))
+    others = lib.dasp_cent_batch_others(batch)
+    if others < 0:
+        _check(others)
+    return out, others
+
+
+def batchFree(batch):
+    _check(library().dasp_cent_batch_free(batch))

@@ -1,11 +1,11 @@
 // batch_kernel.h - the multi-WFS persistent kernel, one instantiation per fused shape.
 //
 // The host lays the WFS of one shape out as a queue of groups (64/NP work items each, a WFS's
-// groups contiguous, items frame-major inside a WFS as in the single-WFS kernel). Waves take
-// groups from a device counter in increasing order, so a WFS boundary is crossed once per wave.
-// The next group's phase and pupil are loaded while the current group is transformed, also
-// across a WFS boundary. The last wave to finish resets the counters, so a captured step graph
-// replays the launch without a memset node.
+// groups contiguous, items frame-major inside a WFS as in the single-WFS kernel). Waves walk the
+// queue in increasing order, statically or by atomic claims (WaveQueue); the next group's phase
+// and pupil are loaded while the current group is transformed, also across a WFS boundary. With
+// claims, the last wave to finish resets the counters, so a captured step graph replays the
+// launch without a memset node.
 //
 // One kernel per shape rather than one for all shapes: a kernel's register allocation covers
 // every shape compiled into it, and a five-shape kernel spills where each single-shape kernel
@@ -21,12 +21,33 @@ __device__ __forceinline__ long wave_uniform(unsigned long long v) {
   return (long)(((unsigned long long)hi << 32) | lo);
 }
 
-// Next queue entry. Called from whole-wave code, so lane 0 is the first active lane.
-__device__ __forceinline__ long grab_group(unsigned long long *ctr, int lane) {
-  unsigned long long g = 0;
-  if (lane == 0) g = atomicAdd(ctr, 1ull);
-  return wave_uniform(g);
-}
+// Work schedule of one wave over the queue. chunk == 0: static, wave w of W takes entries
+// w, w + W, w + 2W, ... (no atomics; balanced when entries cost the same, as they do within one
+// shape). chunk > 0: dynamic, the wave claims chunk consecutive entries per atomic on the
+// queue counter (balances uneven entries; one atomic per chunk bounds the contention on the
+// counter). Called from whole-wave code, so lane 0 is the first active lane.
+struct WaveQueue {
+  unsigned long long *ctr;
+  long stride;   // static: total waves
+  int chunk;
+  long lim = 0;  // dynamic: end of the current claim
+  __device__ long first(long wave) {
+    if (chunk == 0) return wave;
+    return claim();
+  }
+  __device__ long next(long g) {
+    if (chunk == 0) return g + stride;
+    if (g + 1 < lim) return g + 1;
+    return claim();
+  }
+  __device__ long claim() {
+    unsigned long long g = 0;
+    if ((threadIdx.x & 63) == 0) g = atomicAdd(ctr, (unsigned long long)chunk);
+    const long start = wave_uniform(g);
+    lim = start + chunk;
+    return start;
+  }
+};
 
 // Descriptor holding queue entry g: the last one whose first group is <= g.
 __device__ __forceinline__ int desc_of(const BatchArgs &b, long g) {
@@ -50,29 +71,30 @@ __device__ __forceinline__ long desc_end(const BatchArgs &b, int d) {
   return d + 1 < b.ndesc ? b.desc[d + 1].group0 : b.total_groups;
 }
 
-template <int NP, int F, int B, int WAVES, bool FAST>
+template <int NP, int F, int B, int WAVES, bool FAST, bool SPOT>
 __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(NP <= 16 ? 2 : 1)))
-void batch_kernel(BatchArgs b) {
-  constexpr int SPW = 64 / NP, TS = NP + 1;
-  __shared__ cf stage[WAVES][SPW * NP * TS];
+void batch_kernel(BatchArgs b, int chunk) {
+  constexpr int SPW = 64 / NP, SUB = stage_per_subap<NP, F, SPOT>();
+  __shared__ cf stage[WAVES][SPW * SUB];
   const int lane = threadIdx.x & 63;
   const int w = threadIdx.x >> 6;
   const int sl = lane / NP;
   const int j = lane - sl * NP;
-  cf *st = &stage[w][sl * NP * TS];
-  long g = grab_group(b.counters, lane);
+  cf *st = &stage[w][sl * SUB];
+  WaveQueue q{b.counters, (long)gridDim.x * WAVES, chunk};
+  long g = q.first((long)blockIdx.x * WAVES + w);
   if (g < b.total_groups) {
     int d = desc_of(b, g);
     long base = b.desc[d].group0, end = desc_end(b, d);
     PhaseCol<NP> cur;
     load_item<NP>(desc_args(b, d), (g - base) * SPW + sl, j, cur);
     while (true) {
-      const long gn = grab_group(b.counters, lane);   // entries only grow: gn > g
+      const long gn = q.next(g);   // entries only grow: gn > g
       const bool more = gn < b.total_groups;
       const int dn = more && gn >= end ? desc_of(b, gn) : d;
       PhaseCol<NP> nxt;
       if (more) load_item<NP>(desc_args(b, dn), (gn - b.desc[dn].group0) * SPW + sl, j, nxt);
-      fused_item<NP, F, B, FAST>(desc_args(b, d), (g - base) * SPW + sl, j, w, sl, cur, st);
+      fused_item<NP, F, B, FAST, SPOT>(desc_args(b, d), (g - base) * SPW + sl, j, w, sl, cur, st);
       if (!more) break;
       cur = nxt;
       g = gn;
@@ -83,8 +105,8 @@ void batch_kernel(BatchArgs b) {
       }
     }
   }
-  // every wave reaches this point once; the last one resets the queue for the next launch
-  if (lane == 0) {
+  // dynamic schedule: every wave reaches this point once; the last one resets the counters
+  if (chunk > 0 && lane == 0) {
     __threadfence();
     const unsigned long long nw = (unsigned long long)gridDim.x * WAVES;
     if (atomicAdd(b.counters + 1, 1ull) == nw - 1) {
@@ -95,12 +117,12 @@ void batch_kernel(BatchArgs b) {
 }
 
 // Launcher and occupancy query for one shape: waves per workgroup 1, 2 or 4.
-template <int NP, int F, int B>
-hipError_t launch_batch_t(const BatchArgs &a, int waves, int fast, long wgs, hipStream_t st) {
+template <int NP, int F, int B, bool SPOT>
+hipError_t launch_batch_t(const BatchArgs &a, int waves, int chunk, int fast, long wgs, hipStream_t st) {
 #define DASP_BATCH_LAUNCH(W)                                                                           \
   if (waves == W) {                                                                                   \
-    if (fast) hipLaunchKernelGGL((batch_kernel<NP, F, B, W, true>), dim3((unsigned)wgs), dim3(64 * W), 0, st, a); \
-    else hipLaunchKernelGGL((batch_kernel<NP, F, B, W, false>), dim3((unsigned)wgs), dim3(64 * W), 0, st, a);     \
+    if (fast) hipLaunchKernelGGL((batch_kernel<NP, F, B, W, true, SPOT>), dim3((unsigned)wgs), dim3(64 * W), 0, st, a, chunk); \
+    else hipLaunchKernelGGL((batch_kernel<NP, F, B, W, false, SPOT>), dim3((unsigned)wgs), dim3(64 * W), 0, st, a, chunk);     \
     return hipGetLastError();                                                                         \
   }
   DASP_BATCH_LAUNCH(1)
@@ -110,14 +132,14 @@ hipError_t launch_batch_t(const BatchArgs &a, int waves, int fast, long wgs, hip
   return hipErrorInvalidValue;
 }
 
-template <int NP, int F, int B>
+template <int NP, int F, int B, bool SPOT>
 int resident_batch_t(int waves, int fast) {
   int nb = 0;
   hipError_t e = hipErrorInvalidValue;
 #define DASP_BATCH_OCC(W)                                                                                     \
   if (waves == W)                                                                                            \
-    e = fast ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, batch_kernel<NP, F, B, W, true>, 64 * W, 0)   \
-             : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, batch_kernel<NP, F, B, W, false>, 64 * W, 0);
+    e = fast ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, batch_kernel<NP, F, B, W, true, SPOT>, 64 * W, 0)   \
+             : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, batch_kernel<NP, F, B, W, false, SPOT>, 64 * W, 0);
   DASP_BATCH_OCC(1)
   DASP_BATCH_OCC(2)
   DASP_BATCH_OCC(4)

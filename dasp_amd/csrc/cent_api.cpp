@@ -15,6 +15,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <algorithm>
 #include <ctime>
 #include <string>
 #include <vector>
@@ -97,6 +98,7 @@ struct dasp_cent {
   int atmosPhaseType = 0, interpolate = 0;
   int curInteg = 0, form = DASP_CENT_FORM_FFT32;
   int fastTrig = 1;   // reduced-argument hardware sin/cos (max abs error 3.8e-7, measured)
+  unsigned long gen = 0;   // bumped by every update/set_* call: step graphs built on it are stale
   unsigned long seed = 1;
   unsigned long nreadouts = 0;
   float sig = 0.f, skybrightness = 0.f, readnoise = 0.f, readbg = 0.f, noiseFloor = 0.f;
@@ -124,6 +126,7 @@ struct dasp_cent {
   DevBuf d_poisson, d_binA, d_otf, d_corrK, d_corrPat, d_centWeight, d_refCents, d_calData;
   DevBuf d_calBounds, d_calSteps, d_calCoeff, d_ubArr, d_input, d_subapLoc, d_cents, d_bimg;
   DevBuf d_corrimg, d_hll, d_zpois, d_zread, d_bscratch, d_cscratch;
+  DevBuf d_otfk;   // spot kernel table of the fused kernels (psfsize == fftsize)
   std::vector<float> h_stage;
   std::vector<float> noiseHost;
   // kernel timing
@@ -209,6 +212,27 @@ static int set_spot(dasp_cent *c, const dasp_cent_array &sp) {
   }
   if (upload(c->d_otf, otf.data(), otf.size() * sizeof(float2)) != hipSuccess)
     return fail(DASP_CENT_EHIP, "upload of spot OTFs failed");
+  if (P == c->F && P % 2 == 0) {
+    // Table of the fused spot convolution (spot_convolve in fused_item.h), K = conj(OTF):
+    // rows m = 1..P/2-1: K(u, m); row 0: (K(u,0) + K(u,P/2))/2; row P/2: (K(u,0) - K(u,P/2))/2.
+    const int Hh = P / 2;
+    std::vector<float2> tab((size_t)nk * (Hh + 1) * P);
+    for (int k = 0; k < nk; ++k) {
+      const float2 *o = otf.data() + (size_t)k * P * P;
+      float2 *t = tab.data() + (size_t)k * (Hh + 1) * P;
+      for (int u = 0; u < P; ++u) {
+        const float2 k0 = make_float2(o[u * P].x, -o[u * P].y);
+        const float2 kh = make_float2(o[u * P + Hh].x, -o[u * P + Hh].y);
+        t[u] = make_float2(0.5f * (k0.x + kh.x), 0.5f * (k0.y + kh.y));
+        t[(size_t)Hh * P + u] = make_float2(0.5f * (k0.x - kh.x), 0.5f * (k0.y - kh.y));
+        for (int m = 1; m < Hh; ++m) t[(size_t)m * P + u] = make_float2(o[u * P + m].x, -o[u * P + m].y);
+      }
+    }
+    if (upload(c->d_otfk, tab.data(), tab.size() * sizeof(float2)) != hipSuccess)
+      return fail(DASP_CENT_EHIP, "upload of spot kernel table failed");
+  } else {
+    c->d_otfk.release();
+  }
   return 0;
 }
 
@@ -410,8 +434,13 @@ static bool need_read(dasp_cent *c) { return c->readnoise != 0.f && c->calsource
 
 // The fused kernel covers: no spot kernel, clip == fftsize, integer binning of a shape it was
 // instantiated for, phase only, plain or weighted CoG without calibration or pixel selection.
+static bool fused_spot(dasp_cent *c) {
+  return c->spotDim != 0 && c->psf == c->F && c->padded == c->F && c->pbf == 1 && c->F == 2 * pad_pow2(c->n) &&
+         c->d_otfk.p != nullptr;
+}
+
 static bool fused_options(dasp_cent *c) {
-  return c->spotDim == 0 && c->clip == c->F && c->atmosPhaseType == 0 && c->opticalBinning == 0 &&
+  return (c->spotDim == 0 || fused_spot(c)) && c->clip == c->F && c->atmosPhaseType == 0 && c->opticalBinning == 0 &&
          c->imageOnly == 0 && c->correlationCentroiding == 0 && c->parabolicFit == 0 && c->gaussianFit == 0 &&
          !c->haveCalData && !c->haveCalCoeff && (c->calsource != 0 || (c->useBrightest == 0 && !c->haveUBArr));
 }
@@ -419,12 +448,58 @@ static bool fused_options(dasp_cent *c) {
 // The DFT-GEMM kernel covers the fused options for n = 16, fftsize = 32, nimg = 16.
 static bool use_dft(dasp_cent *c) {
   return (c->form == DASP_CENT_FORM_DFT_FP16 || c->form == DASP_CENT_FORM_DFT_BF16) && fused_options(c) &&
-         dft_supported(c->n, c->F, c->nimg);
+         c->spotDim == 0 && dft_supported(c->n, c->F, c->nimg);
 }
 
 static bool use_fused(dasp_cent *c) {
   return c->form != DASP_CENT_FORM_FFT32_GENERAL && fused_options(c) &&
-         (use_dft(c) || fused_supported(pad_pow2(c->n), c->F, c->nimg));
+         (use_dft(c) || fused_supported(pad_pow2(c->n), c->F, c->nimg, c->spotDim != 0));
+}
+
+// Arguments of the fused kernels (FP32 and DFT-GEMM forms) for nframes device frames.
+static void fused_args(dasp_cent *c, const float *d_phs, long phs_stride, int nframes, float *d_cents,
+                       float *d_bimg, const float *d_zpois, const float *d_zread, FusedArgs &fa) {
+  const long nitems = (long)nframes * (long)c->active.size();
+  fa = FusedArgs{};
+  fa.phs = d_phs;
+  fa.phs_stride = phs_stride;
+  fa.pup = c->d_pup.as<float>();
+  fa.tilt = c->d_tilt.as<float>();
+  fa.tiltstep = (float)(M_PI / c->F * (c->F + 1.));
+  fa.fast_trig = c->fastTrig;
+  fa.active = c->d_active.as<int>();
+  fa.nactive = (int)c->active.size();
+  fa.nitems = nitems;
+  fa.nsubx = c->nsubx;
+  fa.n = c->n;
+  fa.npup = c->npup;
+  fa.nimg = c->nimg;
+  fa.ncen = c->ncen;
+  fa.scale = c->d_scale.as<float>();
+  fa.skyadd = c->d_skyadd.as<float>();
+  fa.skyarr = c->haveSkyArr ? c->d_skyarr.as<float>() : nullptr;
+  fa.poisson_on = c->d_poisson.as<unsigned char>();
+  fa.zpois = d_zpois;
+  fa.zread = d_zread;
+  fa.noise_stride = (long)c->nsubaps * c->nimg * c->nimg;
+  fa.readnoise = c->calsource == 0 ? c->readnoise : 0.f;
+  fa.readbg = c->readbg;
+  fa.floor = c->noiseFloor;
+  fa.threshType = c->threshType;
+  fa.calsource = c->calsource;
+  fa.pxlPower = c->pxlPower;
+  fa.centWeight = c->haveCentWeight ? c->d_centWeight.as<float>() : nullptr;
+  fa.centWeightDim = c->haveCentWeight ? c->centWeightDim : 0;
+  fa.refCents = c->haveRef ? c->d_refCents.as<float>() : nullptr;
+  fa.cents = d_cents;
+  fa.cents_stride = (long)c->nsubaps * 2;
+  fa.bimg = d_bimg;
+  fa.bimg_stride = (long)c->nsubaps * c->nimg * c->nimg;
+  fa.slow_detector = (d_zpois && need_pois(c)) || (d_zread && need_read(c)) || c->haveSkyArr ||
+                     c->threshType == 2 || (c->pxlPower != 1.f && c->pxlPower != 0.f) || c->haveCentWeight ||
+                     d_bimg != nullptr;
+  fa.otfk = c->spotDim ? c->d_otfk.as<float2>() : nullptr;
+  fa.otf_per_subap = c->spotDim == 4;
 }
 
 // Phase -> centroids for nframes frames already resident on the device.
@@ -437,48 +512,12 @@ static int launch_phase(dasp_cent *c, const float *d_phs, long phs_stride, int n
   const int np = pad_pow2(c->n);
   const float *amp = c->atmosPhaseType == 1 ? d_phs + (long)c->npup * c->npup : nullptr;
   if (nitems > 0 && use_fused(c)) {
-    FusedArgs fa{};
-    fa.phs = d_phs;
-    fa.phs_stride = phs_stride;
-    fa.pup = c->d_pup.as<float>();
-    fa.tilt = c->d_tilt.as<float>();
-    fa.tiltstep = (float)(M_PI / c->F * (c->F + 1.));
-    fa.fast_trig = c->fastTrig;
-    fa.active = c->d_active.as<int>();
-    fa.nactive = (int)c->active.size();
-    fa.nitems = nitems;
-    fa.nsubx = c->nsubx;
-    fa.n = c->n;
-    fa.npup = c->npup;
-    fa.nimg = c->nimg;
-    fa.ncen = c->ncen;
-    fa.scale = c->d_scale.as<float>();
-    fa.skyadd = c->d_skyadd.as<float>();
-    fa.skyarr = c->haveSkyArr ? c->d_skyarr.as<float>() : nullptr;
-    fa.poisson_on = c->d_poisson.as<unsigned char>();
-    fa.zpois = d_zpois;
-    fa.zread = d_zread;
-    fa.noise_stride = (long)c->nsubaps * c->nimg * c->nimg;
-    fa.readnoise = c->calsource == 0 ? c->readnoise : 0.f;
-    fa.readbg = c->readbg;
-    fa.floor = c->noiseFloor;
-    fa.threshType = c->threshType;
-    fa.calsource = c->calsource;
-    fa.pxlPower = c->pxlPower;
-    fa.centWeight = c->haveCentWeight ? c->d_centWeight.as<float>() : nullptr;
-    fa.centWeightDim = c->haveCentWeight ? c->centWeightDim : 0;
-    fa.refCents = c->haveRef ? c->d_refCents.as<float>() : nullptr;
-    fa.cents = d_cents;
-    fa.cents_stride = (long)c->nsubaps * 2;
-    fa.bimg = d_bimg;
-    fa.bimg_stride = (long)c->nsubaps * c->nimg * c->nimg;
-    fa.slow_detector = (d_zpois && need_pois(c)) || (d_zread && need_read(c)) || c->haveSkyArr ||
-                       c->threshType == 2 || (c->pxlPower != 1.f && c->pxlPower != 0.f) || c->haveCentWeight ||
-                       d_bimg != nullptr;
+    FusedArgs fa;
+    fused_args(c, d_phs, phs_stride, nframes, d_cents, d_bimg, d_zpois, d_zread, fa);
     int trc = timing_mark(c, st);
     if (trc) return trc;
     if (use_dft(c)) HIPCHK(launch_dft(fa, c->form == DASP_CENT_FORM_DFT_BF16, st));
-    else HIPCHK(launch_fused(fa, np, c->F, st));
+    else HIPCHK(launch_fused(fa, np, c->F, c->spotDim != 0, st));
     trc = timing_mark(c, st);
     if (trc) return trc;
   } else if (nitems > 0) {
@@ -588,6 +627,234 @@ static int copy_back(dasp_cent *c) {
 // ------------------------------------------------------------------------------------------------
 // C ABI
 // ------------------------------------------------------------------------------------------------
+// ------------------------------------------------------------------------------------------------
+// device-resident runs
+// ------------------------------------------------------------------------------------------------
+static int check_device_run(dasp_cent *c, const float *d_phs, long phs_stride, int nframes, const float *d_cents,
+                            const float *d_zpois, const float *d_zread) {
+  if (!c) return fail(DASP_CENT_EINVAL, "null handle");
+  if (!d_phs || nframes < 1 || !d_cents) return fail(DASP_CENT_EINVAL, "d_phs, d_cents and nframes >= 1 are required");
+  if (c->nintegrations + c->nlatency != 1)
+    return fail(DASP_CENT_EINVAL, "device runs treat every frame as a readout (nintegrations+nlatency must be 1)");
+  if (phs_stride < (long)c->npup * c->npup * (c->atmosPhaseType + 1)) return fail(DASP_CENT_EINVAL, "phs_stride too small");
+  if (need_pois(c) && !d_zpois) return fail(DASP_CENT_EINVAL, "Poisson noise is on: d_zpois is required");
+  if (need_read(c) && !d_zread) return fail(DASP_CENT_EINVAL, "read noise is on: d_zread is required");
+  return 0;
+}
+
+// Images and correlation images the general path materialises even when the caller wants none.
+static int device_scratch(dasp_cent *c, int nframes, float **bimg, float **corr) {
+  if (!*bimg && !use_fused(c)) {
+    HIPCHK(c->d_bscratch.ensure((size_t)nframes * c->nsubaps * c->nimg * c->nimg * 4));
+    *bimg = c->d_bscratch.as<float>();
+  }
+  if (!*corr && c->correlationCentroiding && c->haveCorr) {
+    HIPCHK(c->d_cscratch.ensure((size_t)nframes * c->nsubaps * c->corrsize * c->corrsize * 4));
+    *corr = c->d_cscratch.as<float>();
+  }
+  if (!use_fused(c) && c->active.size() > 0) {   // image-formation scratch of the general path
+    const long hll = (long)nframes * (long)c->active.size() * c->padded * c->padded;
+    HIPCHK(c->d_hll.ensure(hll * sizeof(float)));
+  }
+  return 0;
+}
+
+// ------------------------------------------------------------------------------------------------
+// multi-WFS step
+// ------------------------------------------------------------------------------------------------
+namespace {
+
+struct BatchLaunch {
+  int np = 0, F = 0, nimg = 0, fast = 0;
+  bool spot = false;
+  std::vector<int> items;      // indices into the batch's item list
+  long groups = 0;
+  DevBuf desc, counters;
+  int waves = 0;
+  long wgs = 0;
+  int chunk = -1;              // -1: not chosen yet
+  float ms = -1.f;
+};
+
+}  // namespace
+
+struct dasp_cent_batch {
+  std::vector<dasp_cent_batch_item> items;
+  std::vector<unsigned long> gens;   // handle generations the graph was built on
+  std::vector<BatchLaunch> launches;
+  std::vector<int> others;           // items running their own kernels inside the graph
+  DevBuf inactive;                   // InactiveDesc[] of the items in persistent launches
+  int ninactive_desc = 0;
+  long inactive_total = 0;
+  int cus = 0;
+  hipStream_t cap = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+
+static bool batch_fusable(dasp_cent *c) {
+  return use_fused(c) && !use_dft(c) && batch_supported(pad_pow2(c->n), c->F, c->nimg, c->spotDim != 0);
+}
+
+static void batch_default_config(dasp_cent_batch *b, BatchLaunch &L) {
+  if (L.chunk < 0) L.chunk = 0;
+  if (L.waves == 0) L.waves = 4;
+  const int per_cu = batch_resident_wgs(L.np, L.F, L.nimg, L.spot, L.waves, L.fast);
+  const long full = (L.groups + L.waves - 1) / L.waves;
+  long w = (long)(per_cu > 0 ? per_cu : 1) * b->cus;
+  if (L.wgs <= 0) L.wgs = w < full ? w : full;
+}
+
+static void batch_drop_graph(dasp_cent_batch *b) {
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  b->exec = nullptr;
+  b->graph = nullptr;
+}
+
+static hipError_t batch_launch(const BatchLaunch &L, int ndesc, int waves, int chunk, long wgs, hipStream_t st) {
+  BatchArgs a;
+  a.desc = L.desc.as<BatchDesc>();
+  a.ndesc = ndesc;
+  a.total_groups = L.groups;
+  a.counters = L.counters.as<unsigned long long>();
+  return launch_batch(a, L.np, L.F, L.nimg, L.spot, waves, chunk, L.fast, wgs, st);
+}
+
+// Descriptors, queue counters and inactive-subaperture lists from the handles' current state.
+// Tuned configurations of launches whose key survives are kept.
+static int batch_layout(dasp_cent_batch *b) {
+  std::vector<BatchLaunch> old;
+  old.swap(b->launches);
+  b->others.clear();
+  b->gens.clear();
+  std::vector<InactiveDesc> ina;
+  long ina_total = 0;
+  for (size_t i = 0; i < b->items.size(); ++i) {
+    const dasp_cent_batch_item &it = b->items[i];
+    dasp_cent *c = it.handle;
+    b->gens.push_back(c->gen);
+    int rc = check_device_run(c, it.phs, it.phs_stride, it.nframes, it.cents, it.zpois, it.zread);
+    if (rc) return fail(rc, "batch item %zu: %s", i, g_err.c_str());
+    if (c->inputDims != 0) return fail(DASP_CENT_EINVAL, "batch item %zu: image-input handles run with run_slope", i);
+    rc = upload_tables(c);
+    if (rc) return rc;
+    if (!batch_fusable(c)) {
+      b->others.push_back((int)i);
+      continue;
+    }
+    const int np = pad_pow2(c->n), fast = c->fastTrig;
+    const bool spot = c->spotDim != 0;
+    BatchLaunch *L = nullptr;
+    for (BatchLaunch &x : b->launches)
+      if (x.np == np && x.F == c->F && x.nimg == c->nimg && x.fast == fast && x.spot == spot) L = &x;
+    if (!L) {
+      b->launches.emplace_back();
+      L = &b->launches.back();
+      L->np = np;
+      L->F = c->F;
+      L->nimg = c->nimg;
+      L->fast = fast;
+      L->spot = spot;
+      for (BatchLaunch &o : old)
+        if (o.np == np && o.F == c->F && o.nimg == c->nimg && o.fast == fast && o.spot == spot) {
+          L->waves = o.waves;
+          L->wgs = o.wgs;
+          L->chunk = o.chunk;
+          L->ms = o.ms;
+        }
+    }
+    L->items.push_back((int)i);
+    if (!c->inactive.empty()) {
+      InactiveDesc q;
+      q.inactive = c->d_inactive.as<int>();
+      q.ninactive = (int)c->inactive.size();
+      q.cents = it.cents;
+      q.cents_stride = (long)c->nsubaps * 2;
+      q.start = ina_total;
+      ina.push_back(q);
+      ina_total += (long)it.nframes * q.ninactive;
+    }
+  }
+  for (BatchLaunch &o : old) {
+    o.desc.release();
+    o.counters.release();
+  }
+  for (BatchLaunch &L : b->launches) {
+    std::vector<BatchDesc> d;
+    const int spw = 64 / L.np;
+    long g0 = 0;
+    for (int i : L.items) {
+      const dasp_cent_batch_item &it = b->items[i];
+      BatchDesc q;
+      fused_args(it.handle, it.phs, it.phs_stride, it.nframes, it.cents, it.bimg, it.zpois, it.zread, q.args);
+      q.group0 = g0;
+      g0 += (q.args.nitems + spw - 1) / spw;
+      d.push_back(q);
+    }
+    L.groups = g0;
+    if (upload(L.desc, d.data(), d.size() * sizeof(BatchDesc))) return fail(DASP_CENT_EHIP, "descriptor upload failed");
+    HIPCHK(L.counters.ensure(2 * sizeof(unsigned long long)));
+    HIPCHK(hipMemset(L.counters.p, 0, 2 * sizeof(unsigned long long)));
+    batch_default_config(b, L);
+  }
+  for (int i : b->others) {
+    dasp_cent_batch_item &it = b->items[i];
+    float *bimg = it.bimg, *corr = it.corrimg;
+    int rc = device_scratch(it.handle, it.nframes, &bimg, &corr);
+    if (rc) return rc;
+  }
+  b->ninactive_desc = (int)ina.size();
+  b->inactive_total = ina_total;
+  if (!ina.empty() && upload(b->inactive, ina.data(), ina.size() * sizeof(InactiveDesc)))
+    return fail(DASP_CENT_EHIP, "descriptor upload failed");
+  return 0;
+}
+
+// Enqueue one step: the persistent launches, the other WFS's own kernels, the inactive slopes.
+static int batch_enqueue(dasp_cent_batch *b, hipStream_t st) {
+  for (const BatchLaunch &L : b->launches) HIPCHK(batch_launch(L, (int)L.items.size(), L.waves, L.chunk, L.wgs, st));
+  for (int i : b->others) {
+    dasp_cent_batch_item &it = b->items[i];
+    dasp_cent *c = it.handle;
+    float *bimg = it.bimg, *corr = it.corrimg;
+    int rc = device_scratch(c, it.nframes, &bimg, &corr);   // sized at layout time: no allocation here
+    if (rc) return rc;
+    const bool timing = c->timing;
+    c->timing = false;   // no event records inside a captured step
+    rc = launch_phase(c, it.phs, it.phs_stride, it.nframes, it.cents, bimg, corr, it.zpois, it.zread, st);
+    c->timing = timing;
+    if (rc) return rc;
+  }
+  HIPCHK(launch_batch_inactive(b->inactive.as<InactiveDesc>(), b->ninactive_desc, b->inactive_total, st));
+  return 0;
+}
+
+static int batch_capture(dasp_cent_batch *b) {
+  batch_drop_graph(b);
+  HIPCHK(hipStreamBeginCapture(b->cap, hipStreamCaptureModeThreadLocal));
+  int rc = batch_enqueue(b, b->cap);
+  hipGraph_t g = nullptr;
+  hipError_t e = hipStreamEndCapture(b->cap, &g);
+  if (rc) {
+    if (g) (void)hipGraphDestroy(g);
+    return rc;
+  }
+  if (e != hipSuccess) return fail(DASP_CENT_EHIP, "step capture failed: %s", hipGetErrorString(e));
+  b->graph = g;
+  HIPCHK(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+  return 0;
+}
+
+static int batch_refresh(dasp_cent_batch *b) {
+  bool stale = !b->exec;
+  for (size_t i = 0; i < b->items.size(); ++i) stale = stale || b->items[i].handle->gen != b->gens[i];
+  if (!stale) return 0;
+  int rc = batch_layout(b);
+  if (rc) return rc;
+  return batch_capture(b);
+}
+
 extern "C" {
 
 const char *dasp_cent_last_error(void) { return g_err.c_str(); }
@@ -789,7 +1056,8 @@ int dasp_cent_free(dasp_cent_t *c) {
                     &c->d_sigv, &c->d_skyadd, &c->d_skyarr, &c->d_poisson, &c->d_binA, &c->d_otf, &c->d_corrK,
                     &c->d_corrPat, &c->d_centWeight, &c->d_refCents, &c->d_calData, &c->d_calBounds,
                     &c->d_calSteps, &c->d_calCoeff, &c->d_ubArr, &c->d_input, &c->d_subapLoc, &c->d_cents,
-                    &c->d_bimg, &c->d_corrimg, &c->d_hll, &c->d_zpois, &c->d_zread, &c->d_bscratch, &c->d_cscratch};
+                    &c->d_bimg, &c->d_corrimg, &c->d_hll, &c->d_zpois, &c->d_zread, &c->d_bscratch, &c->d_cscratch,
+                    &c->d_otfk};
   for (DevBuf *b : bufs) b->release();
   delete c;
   return 0;
@@ -797,9 +1065,10 @@ int dasp_cent_free(dasp_cent_t *c) {
 
 int dasp_cent_set_form(dasp_cent_t *c, int form) {
   if (!c) return fail(DASP_CENT_EINVAL, "null handle");
+  c->gen++;
   if (form < DASP_CENT_FORM_FFT32 || form > DASP_CENT_FORM_FFT32_GENERAL) return fail(DASP_CENT_EINVAL, "unknown form %d", form);
   if ((form == DASP_CENT_FORM_DFT_FP16 || form == DASP_CENT_FORM_DFT_BF16) &&
-      !(fused_options(c) && dft_supported(c->n, c->F, c->nimg)))
+      !(fused_options(c) && c->spotDim == 0 && dft_supported(c->n, c->F, c->nimg)))
     return fail(DASP_CENT_EUNSUPPORTED, "the DFT-GEMM form covers n=16, fftsize=clipsize=32, nimg=16 with a (weighted) CoG, "
                                         "no spot kernel, phase only");
   c->form = form;
@@ -808,6 +1077,7 @@ int dasp_cent_set_form(dasp_cent_t *c, int form) {
 
 int dasp_cent_set_fast_trig(dasp_cent_t *c, int enable) {
   if (!c) return fail(DASP_CENT_EINVAL, "null handle");
+  c->gen++;
   c->fastTrig = enable != 0;
   return 0;
 }
@@ -925,27 +1195,17 @@ int dasp_cent_run_slope(dasp_cent_t *c, float *seconds) {
 int dasp_cent_run_device(dasp_cent_t *c, const float *d_phs, long phs_stride, int nframes, float *d_cents,
                          float *d_bimg, float *d_corrimg, const float *d_zpois, const float *d_zread,
                          void *stream) {
-  if (!c) return fail(DASP_CENT_EINVAL, "null handle");
-  if (!d_phs || nframes < 1 || !d_cents) return fail(DASP_CENT_EINVAL, "d_phs, d_cents and nframes >= 1 are required");
-  if (c->nintegrations + c->nlatency != 1) return fail(DASP_CENT_EINVAL, "run_device treats every frame as a readout (nintegrations+nlatency must be 1)");
-  if (phs_stride < (long)c->npup * c->npup * (c->atmosPhaseType + 1)) return fail(DASP_CENT_EINVAL, "phs_stride too small");
-  if (need_pois(c) && !d_zpois) return fail(DASP_CENT_EINVAL, "Poisson noise is on: d_zpois is required");
-  if (need_read(c) && !d_zread) return fail(DASP_CENT_EINVAL, "read noise is on: d_zread is required");
-  float *bimg = d_bimg;
-  if (!bimg && !use_fused(c)) {   // images not wanted: the general path still materialises them
-    HIPCHK(c->d_bscratch.ensure((size_t)nframes * c->nsubaps * c->nimg * c->nimg * 4));
-    bimg = c->d_bscratch.as<float>();
-  }
-  float *corr = d_corrimg;
-  if (!corr && c->correlationCentroiding && c->haveCorr) {
-    HIPCHK(c->d_cscratch.ensure((size_t)nframes * c->nsubaps * c->corrsize * c->corrsize * 4));
-    corr = c->d_cscratch.as<float>();
-  }
+  int rc = check_device_run(c, d_phs, phs_stride, nframes, d_cents, d_zpois, d_zread);
+  if (rc) return rc;
+  float *bimg = d_bimg, *corr = d_corrimg;
+  rc = device_scratch(c, nframes, &bimg, &corr);
+  if (rc) return rc;
   return launch_phase(c, d_phs, phs_stride, nframes, d_cents, bimg, corr, d_zpois, d_zread, (hipStream_t)stream);
 }
 
 int dasp_cent_update_int(dasp_cent_t *c, int code, long v) {
   if (!c) return fail(DASP_CENT_EINVAL, "null handle");
+  c->gen++;
   switch (code) {
     case DASP_CENT_CALSOURCE: c->calsource = (int)v; break;
     case DASP_CENT_ADDPOISSON: c->addPoisson = (int)v; break;
@@ -983,6 +1243,7 @@ int dasp_cent_update_int(dasp_cent_t *c, int code, long v) {
 
 int dasp_cent_update_float(dasp_cent_t *c, int code, double v) {
   if (!c) return fail(DASP_CENT_EINVAL, "null handle");
+  c->gen++;
   switch (code) {
     case DASP_CENT_SIG:
       c->sig = (float)v;
@@ -1006,6 +1267,7 @@ int dasp_cent_update_float(dasp_cent_t *c, int code, double v) {
 
 int dasp_cent_update_array(dasp_cent_t *c, int code, const dasp_cent_array *va) {
   if (!c || !va) return fail(DASP_CENT_EINVAL, "null argument");
+  c->gen++;
   const dasp_cent_array &a = *va;
   int rc = 0;
   switch (code) {
@@ -1079,6 +1341,7 @@ int dasp_cent_update_array(dasp_cent_t *c, int code, const dasp_cent_array *va) 
 int dasp_cent_update_caldata(dasp_cent_t *c, const dasp_cent_array *cd_, const dasp_cent_array *cb,
                              const dasp_cent_array *cs) {
   if (!c) return fail(DASP_CENT_EINVAL, "null handle");
+  c->gen++;
   if (!cd_ || !cd_->data) {
     c->haveCalData = false;
     c->calNSteps = 0;
@@ -1113,6 +1376,146 @@ int dasp_cent_testcorrelation(const float *img, float *corr, int n) {
   for (size_t q = 0; q < X.size(); ++q) X[q] *= K[q];
   host_dft2(X.data(), n, n, +1);
   for (size_t q = 0; q < X.size(); ++q) corr[q] = (float)X[q].real();
+  return 0;
+}
+
+
+int dasp_cent_batch_create(const dasp_cent_batch_item *items, int nitems, dasp_cent_batch_t **out) {
+  if (!out) return fail(DASP_CENT_EINVAL, "null output pointer");
+  *out = nullptr;
+  if (!items || nitems < 1) return fail(DASP_CENT_EINVAL, "a batch needs at least one item");
+  for (int i = 0; i < nitems; ++i) {
+    if (!items[i].handle) return fail(DASP_CENT_EINVAL, "batch item %d: null handle", i);
+    for (int j = 0; j < i; ++j)
+      if (items[j].handle == items[i].handle) return fail(DASP_CENT_EINVAL, "batch items %d and %d share a handle", j, i);
+  }
+  dasp_cent_batch *b = new dasp_cent_batch;
+  b->items.assign(items, items + nitems);
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e == hipSuccess) e = hipDeviceGetAttribute(&b->cus, hipDeviceAttributeMultiprocessorCount, dev);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&b->cap, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    dasp_cent_batch_free(b);
+    return fail(DASP_CENT_EHIP, "batch set-up: %s", hipGetErrorString(e));
+  }
+  int rc = batch_layout(b);
+  if (!rc) rc = batch_capture(b);
+  if (rc) {
+    std::string msg = g_err;
+    dasp_cent_batch_free(b);
+    return fail(rc, "%s", msg.c_str());
+  }
+  *out = b;
+  return 0;
+}
+
+int dasp_cent_batch_tune(dasp_cent_batch_t *b, void *stream, int reps) {
+  if (!b) return fail(DASP_CENT_EINVAL, "null batch");
+  if (reps < 1) reps = 1;
+  int rc = batch_refresh(b);
+  if (rc) return rc;
+  hipStream_t st = (hipStream_t)stream;
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  rc = 0;
+  for (BatchLaunch &L : b->launches) {
+    float best = -1.f;
+    int bw = L.waves, bc = L.chunk;
+    long bg = L.wgs;
+    struct Cand {
+      int waves, chunk;
+      long wgs;
+    };
+    std::vector<Cand> cand;
+    for (int waves : {1, 2, 4}) {
+      const int per_cu = batch_resident_wgs(L.np, L.F, L.nimg, L.spot, waves, L.fast);
+      const long full = std::min((L.groups + waves - 1) / waves, 0x7fffffffL);
+      for (int k = 1; k <= per_cu; ++k) cand.push_back({waves, 0, std::min((long)k * b->cus, full)});
+      cand.push_back({waves, 0, full});   // one group per wave: no persistence
+      for (int chunk : {2, 8}) cand.push_back({waves, chunk, std::min((long)std::max(per_cu, 1) * b->cus, full)});
+    }
+    for (const Cand &cd : cand) {
+      hipError_t e = batch_launch(L, (int)L.items.size(), cd.waves, cd.chunk, cd.wgs, st);   // warm-up
+      if (e == hipSuccess) e = hipEventRecord(e0, st);
+      for (int r = 0; r < reps && e == hipSuccess; ++r) e = batch_launch(L, (int)L.items.size(), cd.waves, cd.chunk, cd.wgs, st);
+      if (e == hipSuccess) e = hipEventRecord(e1, st);
+      if (e == hipSuccess) e = hipEventSynchronize(e1);
+      float ms = 0.f;
+      if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+      if (e != hipSuccess) {
+        rc = fail(DASP_CENT_EHIP, "tuning launch failed: %s", hipGetErrorString(e));
+        break;
+      }
+      ms /= reps;
+      if (best < 0.f || ms < best) {
+        best = ms;
+        bw = cd.waves;
+        bc = cd.chunk;
+        bg = cd.wgs;
+      }
+    }
+    if (rc) break;
+    L.waves = bw;
+    L.chunk = bc;
+    L.wgs = bg;
+    L.ms = best;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (rc) return rc;
+  return batch_capture(b);
+}
+
+int dasp_cent_batch_run(dasp_cent_batch_t *b, void *stream) {
+  if (!b) return fail(DASP_CENT_EINVAL, "null batch");
+  int rc = batch_refresh(b);
+  if (rc) return rc;
+  HIPCHK(hipGraphLaunch(b->exec, (hipStream_t)stream));
+  return 0;
+}
+
+int dasp_cent_batch_launches(dasp_cent_batch_t *b) {
+  if (!b) return fail(DASP_CENT_EINVAL, "null batch");
+  return (int)b->launches.size();
+}
+
+int dasp_cent_batch_launch_info(dasp_cent_batch_t *b, int i, int shape[4], int *nwfs, long *groups,
+                                int *waves_per_wg, long *workgroups, int *chunk, float *ms) {
+  if (!b) return fail(DASP_CENT_EINVAL, "null batch");
+  if (i < 0 || i >= (int)b->launches.size()) return fail(DASP_CENT_EINVAL, "launch %d out of range", i);
+  const BatchLaunch &L = b->launches[i];
+  if (shape) {
+    shape[0] = L.np;
+    shape[1] = L.F;
+    shape[2] = L.nimg;
+    shape[3] = L.spot;
+  }
+  if (nwfs) *nwfs = (int)L.items.size();
+  if (groups) *groups = L.groups;
+  if (waves_per_wg) *waves_per_wg = L.waves;
+  if (workgroups) *workgroups = L.wgs;
+  if (chunk) *chunk = L.chunk;
+  if (ms) *ms = L.ms;
+  return 0;
+}
+
+int dasp_cent_batch_others(dasp_cent_batch_t *b) {
+  if (!b) return fail(DASP_CENT_EINVAL, "null batch");
+  return (int)b->others.size();
+}
+
+int dasp_cent_batch_free(dasp_cent_batch_t *b) {
+  if (!b) return 0;
+  batch_drop_graph(b);
+  for (BatchLaunch &L : b->launches) {
+    L.desc.release();
+    L.counters.release();
+  }
+  b->inactive.release();
+  if (b->cap) (void)hipStreamDestroy(b->cap);
+  delete b;
   return 0;
 }
 

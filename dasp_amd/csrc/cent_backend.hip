@@ -740,4 +740,27 @@ hipError_t launch_inactive(const InactiveArgs &a, hipStream_t st) {
   return hipGetLastError();
 }
 
+// Zero slopes of the inactive subapertures of several WFS in one launch (multi-WFS step).
+__global__ void batch_inactive_kernel(const InactiveDesc *d, int nd, long total) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= total) return;
+  int lo = 0, hi = nd - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (d[mid].start <= t) lo = mid;
+    else hi = mid - 1;
+  }
+  const InactiveDesc q = d[lo];
+  const long it = t - q.start;
+  const int frame = (int)(it / q.ninactive);
+  const int s = q.inactive[it - (long)frame * q.ninactive];
+  *reinterpret_cast<float2 *>(q.cents + (long)frame * q.cents_stride + s * 2) = make_float2(0.f, 0.f);
+}
+
+hipError_t launch_batch_inactive(const InactiveDesc *d, int nd, long total, hipStream_t st) {
+  if (total <= 0 || nd <= 0) return hipSuccess;
+  hipLaunchKernelGGL(batch_inactive_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, d, nd, total);
+  return hipGetLastError();
+}
+
 }  // namespace dasp

@@ -5,9 +5,9 @@ namespace dasp {
 
 #define DASP_BATCH_LARGE(X) X(32, 64, 2) X(32, 64, 4) X(32, 64, 8)
 
-hipError_t launch_batch32(const BatchArgs &a, int F, int b, int waves, int fast, long wgs, hipStream_t st) {
+hipError_t launch_batch32(const BatchArgs &a, int F, int b, int waves, int chunk, int fast, long wgs, hipStream_t st) {
 #define DASP_BATCH_CASE(A, Bf, Bb) \
-  if (F == Bf && b == Bb) return launch_batch_t<A, Bf, Bb>(a, waves, fast, wgs, st);
+  if (F == Bf && b == Bb) return launch_batch_t<A, Bf, Bb, false>(a, waves, chunk, fast, wgs, st);
   DASP_BATCH_LARGE(DASP_BATCH_CASE)
 #undef DASP_BATCH_CASE
   return hipErrorInvalidValue;
@@ -15,7 +15,7 @@ hipError_t launch_batch32(const BatchArgs &a, int F, int b, int waves, int fast,
 
 int resident_batch32(int F, int b, int waves, int fast) {
 #define DASP_BATCH_OCC(A, Bf, Bb) \
-  if (F == Bf && b == Bb) return resident_batch_t<A, Bf, Bb>(waves, fast);
+  if (F == Bf && b == Bb) return resident_batch_t<A, Bf, Bb, false>(waves, fast);
   DASP_BATCH_LARGE(DASP_BATCH_OCC)
 #undef DASP_BATCH_OCC
   return 0;

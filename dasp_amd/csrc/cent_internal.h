@@ -140,6 +140,10 @@ struct FusedArgs {
   long cents_stride;
   float *bimg;           // may be nullptr (images not returned)
   long bimg_stride;
+  // LGS spot kernel (psfsize == clipsize == fftsize == 2 * phase pad): per subaperture
+  // (F/2 + 1) rows of F complex, see spot_convolve in fused_item.h
+  const float2 *otfk;
+  int otf_per_subap;
 };
 
 // Multi-WFS persistent step (batch_kernel.h): one descriptor per WFS in a shape's launch.
@@ -155,17 +159,28 @@ struct BatchArgs {
   unsigned long long *counters;   // [0] next queue entry, [1] waves finished; zero between launches
 };
 
-bool batch_supported(int n_pad, int F, int nimg);
-int batch_resident_wgs(int n_pad, int F, int nimg, int waves_per_wg, int fast);   // per CU
-hipError_t launch_batch(const BatchArgs &a, int n_pad, int F, int nimg, int waves_per_wg, int fast,
-                        long workgroups, hipStream_t st);
+// Inactive subapertures of one WFS in a multi-WFS step: entries [start, start + nframes*ninactive).
+struct InactiveDesc {
+  const int *inactive;
+  int ninactive;
+  float *cents;
+  long cents_stride;
+  long start;
+};
+hipError_t launch_batch_inactive(const InactiveDesc *d, int nd, long total, hipStream_t st);
+
+bool batch_supported(int n_pad, int F, int nimg, bool spot);
+int batch_resident_wgs(int n_pad, int F, int nimg, bool spot, int waves_per_wg, int fast);   // per CU
+// chunk: 0 = static schedule, > 0 = entries claimed per atomic (see WaveQueue in batch_kernel.h)
+hipError_t launch_batch(const BatchArgs &a, int n_pad, int F, int nimg, bool spot, int waves_per_wg, int chunk,
+                        int fast, long workgroups, hipStream_t st);
 
 hipError_t launch_form(const FormArgs &a, int n_pad, int F, hipStream_t st);
 hipError_t launch_backend(const BackArgs &a, hipStream_t st);
 hipError_t launch_inactive(const InactiveArgs &a, hipStream_t st);
 hipError_t launch_fill(float *p, long n, float v, hipStream_t st);
-bool fused_supported(int n_pad, int F, int nimg);
-hipError_t launch_fused(const FusedArgs &a, int n_pad, int F, hipStream_t st);
+bool fused_supported(int n_pad, int F, int nimg, bool spot);
+hipError_t launch_fused(const FusedArgs &a, int n_pad, int F, bool spot, hipStream_t st);
 bool dft_supported(int n, int F, int nimg);
 hipError_t launch_dft(const FusedArgs &a, int bf16, hipStream_t st);
 

@@ -130,6 +130,16 @@ __device__ __forceinline__ void fft_fwd(cf (&x)[N]) {
   }
 }
 
+// In-place unnormalised inverse FFT, X[k] = sum_i x[i] exp(+2 pi i k i / N) (the FFTW_BACKWARD
+// convention): swapping real and imaginary parts turns the forward transform into the inverse,
+// and the swaps are register renames.
+template <int N>
+__device__ __forceinline__ void fft_inv(cf (&x)[N]) {
+  static_for<0, N>([&](auto i) { x[decltype(i)::value] = {x[decltype(i)::value].y, x[decltype(i)::value].x}; });
+  fft_fwd<N>(x);
+  static_for<0, N>([&](auto i) { x[decltype(i)::value] = {x[decltype(i)::value].y, x[decltype(i)::value].x}; });
+}
+
 // Zero-padded transform: x holds NP inputs (x[i], i<NP; the rest of the F-point input is zero);
 // out[k] = sum_{i<NP} x[i] exp(-2 pi i k i / F) for k < F, with F = R*NP.
 // out[R*q + r] = FFT_NP(x[i] * w_F^{i r})[q].

@@ -94,8 +94,103 @@ __device__ __forceinline__ void load_item(const FusedArgs &a, long item, int j, 
   });
 }
 
+// LDS complex values per subaperture: the pass-1 transpose (NP rows of NP + 1), or with a spot
+// kernel the F x (NP + 1) spectrum transpose plus one value so consecutive subapertures start
+// on different banks.
+template <int NP, int F, bool SPOT>
+constexpr int stage_per_subap() {
+  return SPOT ? F * (NP + 1) + 1 : NP * (NP + 1);
+}
+
+// LGS spot convolution, doConvolution (cmod/src/centmodule.c:524-564): the reference convolves
+// the 180-degree rotated |X|^2 (hll) with the quadrant-swapped spot through r2c/c2r transforms
+// (unnormalised, so the result carries psfsize^2, absorbed by the theoretical total). With
+// R a -> a(F-1-.) in both axes, conv(R a, g) = R conv(a, g(-.)) and the spectrum of g(-.) is
+// conj(OTF) for a real g: the unrotated |X|^2 of the lane's rows k0 = 2j, k1 = 2j+1 (z[l].x,
+// z[l].y) is convolved with conj(OTF) and the rotated binning map is kept.
+//   rows:    one F-point FFT of the packed pair, Hermitian unpack in registers (v = 0..F/2);
+//   columns: through LDS, lane m owns column v = m (m >= 1) or the packed pair v = 0 (re) and
+//            v = F/2 (im) (m = 0), one F-point FFT, times the kernel, one inverse FFT;
+//   rows:    back through LDS, packed inverse F-point FFT of the two Hermitian row spectra.
+// Kernel table per subaperture (host-built from OTF): row m < NP holds A_m(u) = conj OTF(u, m)
+// for m >= 1 and (K(u,0) + K(u,F/2))/2 for m = 0; row NP holds (K(u,0) - K(u,F/2))/2, with
+// K = conj OTF. Lane m = 0 forms A_0 G(u) + B conj(G(-u)), which is K(u,0) S0(u) + i K(u,F/2)
+// SF(u) for the two Hermitian columns S0, SF packed in G.
+template <int NP, int F, int B, int RB, int NIMG>
+__device__ __forceinline__ void spot_convolve(const FusedArgs &a, int s, int j, cf (&z)[F], cf *T,
+                                              float (&acc)[RB][NIMG]) {
+  static_assert(F == 2 * NP, "one column per lane needs F == 2 NP");
+  constexpr int TS2 = NP + 1, H = F / 2;
+  const int k0 = 2 * j;
+  fft_fwd<F>(z);
+  cf p0{0.f, 0.f}, p1{0.f, 0.f};
+  static_for<0, H + 1>([&](auto vv) {
+    constexpr int v = decltype(vv)::value;
+    const cf zv = z[v], zc = z[(F - v) % F];
+    const cf h0 = {0.5f * (zv.x + zc.x), 0.5f * (zv.y - zc.y)};    // row k0: (zv + conj zc) / 2
+    const cf h1 = {0.5f * (zv.y + zc.y), -0.5f * (zv.x - zc.x)};   // row k1: (zv - conj zc) / 2i
+    if constexpr (v == 0) {
+      p0.x = h0.x;
+      p1.x = h1.x;
+    } else if constexpr (v == H) {
+      p0.y = h0.x;
+      p1.y = h1.x;
+      T[k0 * TS2] = p0;
+      T[(k0 + 1) * TS2] = p1;
+    } else {
+      T[k0 * TS2 + v] = h0;
+      T[(k0 + 1) * TS2 + v] = h1;
+    }
+  });
+  wave_lds_sync();
+  cf c[F];
+  static_for<0, F>([&](auto kk) { c[decltype(kk)::value] = T[decltype(kk)::value * TS2 + j]; });
+  wave_lds_sync();
+  const float2 *kp = a.otfk + (a.otf_per_subap ? (long)s : 0L) * ((NP + 1) * F);
+  fft_fwd<F>(c);
+  cf o[F];
+  static_for<0, H + 1>([&](auto uu) {
+    constexpr int u = decltype(uu)::value, um = (F - u) % F;
+    const cf gu = c[u], gm = c[um];
+    const float2 au = kp[j * F + u], am = kp[j * F + um];
+    cf ou = cmul(gu, {au.x, au.y});
+    cf om = cmul(gm, {am.x, am.y});
+    if (j == 0) {
+      const float2 bu = kp[NP * F + u], bm = kp[NP * F + um];
+      ou = cadd(ou, cmul({gm.x, -gm.y}, {bu.x, bu.y}));
+      om = cadd(om, cmul({gu.x, -gu.y}, {bm.x, bm.y}));
+    }
+    o[u] = ou;
+    if constexpr (um != u) o[um] = om;
+  });
+  fft_inv<F>(o);
+  static_for<0, F>([&](auto kk) { T[decltype(kk)::value * TS2 + j] = o[decltype(kk)::value]; });
+  wave_lds_sync();
+  cf q[F];
+  {
+    const cf r0 = T[k0 * TS2], r1 = T[(k0 + 1) * TS2];
+    q[0] = {r0.x, r1.x};
+    q[H] = {r0.y, r1.y};
+  }
+  static_for<1, H>([&](auto vv) {
+    constexpr int v = decltype(vv)::value;
+    const cf r0 = T[k0 * TS2 + v], r1 = T[(k0 + 1) * TS2 + v];
+    q[v] = {r0.x - r1.y, r0.y + r1.x};         // R0 + i R1
+    q[F - v] = {r0.x + r1.y, r1.x - r0.y};     // conj R0 + i conj R1
+  });
+  wave_lds_sync();
+  fft_inv<F>(q);
+  constexpr int R = F / NP;
+  constexpr int ib0 = R >= B ? 0 / B : 0, ib1 = R >= B ? 1 / B : 0;
+  static_for<0, F>([&](auto ll) {
+    constexpr int l = decltype(ll)::value, jb = (F - 1 - l) / B;
+    acc[ib0][jb] += q[l].x;
+    acc[ib1][jb] += q[l].y;
+  });
+}
+
 // One work item (frame, subaperture) per NP-lane group: image formation, detector, CoG.
-template <int NP, int F, int B, bool FAST>
+template <int NP, int F, int B, bool FAST, bool SPOT = false>
 __device__ __forceinline__ void fused_item(const FusedArgs &a, long item, int j, int w, int sl,
                                            const PhaseCol<NP> &col, cf *st) {
   using S = FusedShape<NP, F, B>;
@@ -124,6 +219,7 @@ __device__ __forceinline__ void fused_item(const FusedArgs &a, long item, int j,
   static_for<0, RB>([&](auto ib) {
     static_for<0, NIMG>([&](auto jb) { acc[decltype(ib)::value][decltype(jb)::value] = 0.f; });
   });
+  cf zs[SPOT ? F : 1];   // spot kernel: |X|^2 of rows 2j (re) and 2j+1 (im), unrotated
   static_for<0, R>([&](auto rr) {
     constexpr int r = decltype(rr)::value;
     {
@@ -152,10 +248,16 @@ __device__ __forceinline__ void fused_item(const FusedArgs &a, long item, int j,
       static_for<0, NP>([&](auto qq) {
         constexpr int l = R * decltype(qq)::value + r2;
         constexpr int jb = (F - 1 - l) / B;
-        acc[ib][jb] += cabs2(y2[decltype(qq)::value]);
+        if constexpr (SPOT) {
+          if constexpr (r == 0) zs[l].x = cabs2(y2[decltype(qq)::value]);
+          else zs[l].y = cabs2(y2[decltype(qq)::value]);
+        } else {
+          acc[ib][jb] += cabs2(y2[decltype(qq)::value]);
+        }
       });
     });
   });
+  if constexpr (SPOT) spot_convolve<NP, F, B, RB, NIMG>(a, s, j, zs, st, acc);
   if constexpr (G > 1) {
     static_for<0, RB>([&](auto ib) {
       static_for<0, NIMG>([&](auto jb) {

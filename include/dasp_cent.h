@@ -188,6 +188,55 @@ int dasp_cent_info(dasp_cent_t *h, int *nactive, int *npup, int *corrsize);
 int dasp_cent_set_timing(dasp_cent_t *h, int enable);
 int dasp_cent_kernel_time(dasp_cent_t *h, int reset, double *ms, long *launches);
 
+/* ---- multi-WFS step (no counterpart in the reference: science/wfscent.py runs each WFS's
+ * centcmod in turn, one host-threaded call per WFS and frame) -------------------------------- */
+
+typedef struct dasp_cent_batch dasp_cent_batch_t;
+
+/* One WFS of a multi-WFS step: a handle and its device buffers, laid out as for
+ * dasp_cent_run_device (cents (nframes, nsubaps, 2); bimg, corrimg, zpois, zread optional). */
+typedef struct {
+  dasp_cent_t *handle;
+  const float *phs;
+  long phs_stride;
+  int nframes;
+  float *cents;
+  float *bimg;
+  float *corrimg;
+  const float *zpois;
+  const float *zread;
+} dasp_cent_batch_item;
+
+/* Build a step over nitems WFS (each handle at most once). WFS whose options the fused kernels
+ * cover, in the FFT32 form, run in one persistent work-queue launch per (fftsize, phase pad, nimg,
+ * trig mode); the others run their own kernels inside the same step graph. The buffers are
+ * captured: a step always reads and writes the same device addresses. A handle changed by
+ * update/set_form/set_fast_trig after this call is picked up at the next dasp_cent_batch_run
+ * (the step graph is rebuilt). The handles must outlive the batch. */
+int dasp_cent_batch_create(const dasp_cent_batch_item *items, int nitems, dasp_cent_batch_t **out);
+
+/* Autotuner: for every persistent launch, time each (waves per workgroup in {1,2,4}) x
+ * (static schedule with workgroups = k x CUs, k up to the resident limit, or one group per wave;
+ * dynamic schedule claiming 2 or 8 groups per atomic at the resident limit) with HIP events over
+ * reps launches on stream, keep the fastest, then rebuild the step graph. Synchronous. */
+int dasp_cent_batch_tune(dasp_cent_batch_t *b, void *stream, int reps);
+
+/* Run one step (a single hipGraphLaunch on stream; asynchronous). */
+int dasp_cent_batch_run(dasp_cent_batch_t *b, void *stream);
+
+/* Number of persistent launches in the step, and the configuration of launch i: its shape
+ * (phase pad NP, fftsize, nimg, 1 if the WFS convolve with a spot kernel), number of WFS, queue length in groups of 64/NP work items,
+ * waves per workgroup, workgroups, schedule (0 static, c > 0 dynamic claims of c groups) and
+ * the tuned time in ms (-1 before tuning). */
+int dasp_cent_batch_launches(dasp_cent_batch_t *b);
+int dasp_cent_batch_launch_info(dasp_cent_batch_t *b, int i, int shape[4], int *nwfs, long *groups,
+                                int *waves_per_wg, long *workgroups, int *chunk, float *ms);
+
+/* Number of WFS that run outside the persistent launches (their own kernels in the graph). */
+int dasp_cent_batch_others(dasp_cent_batch_t *b);
+
+int dasp_cent_batch_free(dasp_cent_batch_t *b);
+
 #ifdef __cplusplus
 }
 #endif
