@@ -33,7 +33,38 @@ def algorithmic_bytes(w, nactive, frames, images, noise):
     return frames * nactive * per + nactive * 4 * w.n * w.n
 
 
-def cpu_baseline(w, phs0, budget_s=10.0):
+def measured_traffic(workload, form):
+    """HBM bytes per subaperture image of the main kernel, from the rocprofv3 FETCH_SIZE and
+    WRITE_SIZE runs recorded under profiles/ (lines 'traffic: <workload> <form> <bytes>'), or None."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic.md")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        for line in f:
+            p = line.split()
+            if len(p) == 4 and p[0] == "traffic:" and p[1] == workload and p[2] == form:
+                return float(p[3])
+    return None
+
+
+def roofline(form, w, nact, frames, kavg, abytes, noise):
+    """Roofline entry of the main kernel: HBM for the FP32 form, MFMA for the DFT-GEMM forms."""
+    ach = abytes / kavg / 1e9
+    per = measured_traffic(w.name, form)
+    traffic = per * nact * frames if per is not None and not noise else None
+    if form == "fft32":
+        return {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
+                "traffic": traffic, "kernel_avg_ms": kavg * 1e3, "algorithmic_bytes": abytes}
+    # DFT-GEMM form: 18 v_mfma_f32_32x32x16 per subaperture image (hi/lo split, 3 products)
+    flops = 18 * 2 * 32 * 32 * 16 * nact * frames
+    tf = flops / kavg / 1e12
+    return {"bound": "mfma", "achieved": tf, "peak": MFMA_PEAK_TFLOPS[form], "unit": "TFLOP/s",
+            "frac": tf / MFMA_PEAK_TFLOPS[form], "traffic": traffic, "kernel_avg_ms": kavg * 1e3,
+            "mfma_flops": flops, "useful_flops": 8 * (w.nfft * w.n ** 2 + w.nfft ** 2 * w.n) * nact * frames,
+            "hbm_achieved_gbs": ach, "algorithmic_bytes": abytes}
+
+
+def cpu_baseline(w, phs0, budget_s=10.0, corr=None):
     """CPU leg: the FP32 C restatement of the reference engine (oracle/cref), threaded like the
     reference's pthread pool over this host's cores (at most 16), timed on repeated frames of the
     same workload until the budget is spent; then the FP64 oracle on frame 0 for the error figure.
@@ -47,7 +78,8 @@ def cpu_baseline(w, phs0, budget_s=10.0):
     frac = (pup.getSubarea(w.nsubx) / (w.n * w.n)).astype(np.float32)
     nact = int(flag.sum())
     nthr = max(1, min(16, os.cpu_count() or 1))
-    cr = CRef(w.nsubx, w.n, w.nfft, w.clip, w.nimg, w.cen, pupfn.reshape(-1, w.n, w.n), flag, frac, sig=float(w.sig))
+    cr = CRef(w.nsubx, w.n, w.nfft, w.clip, w.nimg, w.cen, pupfn.reshape(-1, w.n, w.n), flag, frac, sig=float(w.sig),
+              corrPattern=corr, parabolicFit=w.parabolicFit if corr is not None else 0)
     cr.run(phs0, nthreads=nthr)
     t0 = time.perf_counter()
     frames = 0
@@ -57,7 +89,9 @@ def cpu_baseline(w, phs0, budget_s=10.0):
         if time.perf_counter() - t0 > budget_s:
             break
     dt = time.perf_counter() - t0
-    p = O.Params(w.nsubx, w.n, w.nfft, w.clip, w.nimg, w.cen, pupfn, flag, frac, sig=float(w.sig))
+    p = O.Params(w.nsubx, w.n, w.nfft, w.clip, w.nimg, w.cen, pupfn, flag, frac, sig=float(w.sig),
+                 correlationCentroiding=1 if corr is not None else 0, corrPattern=corr,
+                 parabolicFit=w.parabolicFit if corr is not None else 0)
     oc, _, _ = O.run_phase(p, phs0)
     sample = ("%d frames x %d active subapertures of %s, FP32 C restatement of centmodule.c (own radix-2 FFT, "
               "no FFTW), %d threads, %.1f s" % (frames, nact, w.name, nthr, dt))
@@ -71,11 +105,15 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--form", default="fft32", choices=["fft32", "bf16", "fp16"])
     ap.add_argument("--workload", default="c3", choices=["c1", "c2", "c3", "c5", "c4"])
+    ap.add_argument("--tune-cache", default=None,
+                    help="JSON file of autotuned multi-WFS launch configurations (read if present, else written)")
     ap.add_argument("--batch", action="store_true",
                     help="run the single WFS through the multi-WFS step (persistent queue kernel + graph)")
     ap.add_argument("--frames", type=int, default=0, help="frames per step (default: the workload's)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--accurate-trig", action="store_true", help="sincosf instead of the reduced hardware sin/cos")
+    ap.add_argument("--no-compare-forms", dest="compare_forms", action="store_false",
+                    help="skip timing the other compute forms (80x80 nfft=32 only)")
     ap.add_argument("--calib-mib", type=int, default=0,
                     help="after timing, copy a buffer of this many MiB once (known-size kernel for scaling "
                          "rocprofv3 FETCH_SIZE/WRITE_SIZE counters)")
@@ -132,49 +170,62 @@ def main():
     wb = None
     if args.batch:
         from dasp_amd.wfsbatch import WFSBatch
-        wb = WFSBatch([cc], frames)
-        wb.phs[0].copy_(phs)
-        d_cents = wb.cents[0].view(frames, S, 2)
         if noise:
             raise SystemExit("--batch: the noise arrays of the multi-WFS step are filled by WFSBatch.draw_noise")
+        wb = WFSBatch([cc], frames)
+        wb.phs[0].copy_(phs)
         wb.tune(reps=3)
 
-    def step():
-        if wb is not None:
-            wb.run()
-            return
-        cent.runDevice(h, phs.data_ptr(), npup * npup, frames, d_cents.data_ptr(), 0,
-                       d_corr.data_ptr() if d_corr is not None else 0,
-                       zp.data_ptr() if zp is not None else 0, zr.data_ptr() if zr is not None else 0,
-                       stream.cuda_stream)
+    def run_form(form):
+        """Time args.steps steps of one compute form; returns (ms per step, kernel avg s, slopes of frame 0)."""
+        cent.setForm(h, {"fft32": 0, "bf16": 1, "fp16": 2}[form])
+        cents = wb.cents[0].view(frames, S, 2) if wb is not None else d_cents
 
-    for _ in range(args.warmup):
-        step()
-    torch.cuda.synchronize()
-    cent.setTiming(h, True)
-    cent.kernelTime(h, reset=True)
-    e0 = torch.cuda.Event(enable_timing=True)
-    e1 = torch.cuda.Event(enable_timing=True)
-    torch.cuda.synchronize()
+        def step():
+            if wb is not None:
+                wb.run()
+                return
+            cent.runDevice(h, phs.data_ptr(), npup * npup, frames, d_cents.data_ptr(), 0,
+                           d_corr.data_ptr() if d_corr is not None else 0,
+                           zp.data_ptr() if zp is not None else 0, zr.data_ptr() if zr is not None else 0,
+                           stream.cuda_stream)
+
+        for _ in range(args.warmup):
+            step()
+        torch.cuda.synchronize()
+        cent.setTiming(h, True)
+        cent.kernelTime(h, reset=True)
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        e0.record(stream)
+        for _ in range(args.steps):
+            step()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / args.steps
+        kms, klaunch = cent.kernelTime(h, reset=True)
+        cent.setTiming(h, False)
+        if wb is not None:
+            launches, _ = wb.info()
+            kms, klaunch = launches[0]["tuned_ms"], 1
+        kavg = kms / klaunch / 1e3 if klaunch > 0 else None
+        return ms, kavg, cents[0].cpu().numpy().reshape(-1, 2)
+
     t0 = time.perf_counter()
-    e0.record(stream)
-    for _ in range(args.steps):
-        step()
-    e1.record(stream)
-    torch.cuda.synchronize()
+    ms_step, kavg, c0 = run_form(args.form)
     wall = time.perf_counter() - t0
-    ms = e0.elapsed_time(e1)
-    kms, klaunch = cent.kernelTime(h, reset=True)
-    cent.setTiming(h, False)
-    ms_step = ms / args.steps
     if args.calib_mib:
+        # a plain elementwise pass of known size for scaling the FETCH_SIZE/WRITE_SIZE counters:
+        # calib_mib MiB read and calib_mib MiB written by one negation kernel
         src = torch.ones(args.calib_mib * (1 << 20) // 4, dtype=torch.float32, device=dev)
         dst = torch.empty_like(src)
         torch.cuda.synchronize()
-        dst.copy_(src)
+        torch.neg(src, out=dst)
         torch.cuda.synchronize()
         del src, dst
-    value = nact * frames * args.steps / (ms / 1e3)
+    value = nact * frames / (ms_step / 1e3)
+    abytes = algorithmic_bytes(w, nact, frames, False, noise)
 
     result = {
         "metric": "subap images+centroids/s @80x80 nfft=32, plain vs MFMA-DFT; RMS err vs FP64",
@@ -194,37 +245,39 @@ def main():
                    "noise": noise, "images_returned": False, "fast_trig": not args.accurate_trig, "wall_s": wall},
     }
     if wb is not None:
-        launches, _ = wb.info()
-        kms, klaunch = launches[0]["tuned_ms"], 1
-        result["config"]["path"] = "multi-WFS step (1 WFS): %s" % launches[0]
-    if klaunch > 0:
-        kavg = kms / klaunch / 1e3
-        abytes = algorithmic_bytes(w, nact, frames, False, noise)
-        ach = abytes / kavg / 1e9
-        if args.form == "fft32":
-            result["roofline"] = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                                  "frac": ach / HBM_PEAK_GBS, "traffic": None,
-                                  "kernel_avg_ms": kavg * 1e3, "algorithmic_bytes": abytes}
-        else:
-            # DFT-GEMM form: 18 v_mfma_f32_32x32x16 per subaperture image (hi/lo, 3 products)
-            flops = 18 * 2 * 32 * 32 * 16 * nact * frames
-            tf = flops / kavg / 1e12
-            result["roofline"] = {"bound": "mfma", "achieved": tf, "peak": MFMA_PEAK_TFLOPS[args.form],
-                                  "unit": "TFLOP/s", "frac": tf / MFMA_PEAK_TFLOPS[args.form], "traffic": None,
-                                  "kernel_avg_ms": kavg * 1e3, "mfma_flops": flops,
-                                  "useful_flops": 8 * (w.nfft * w.n ** 2 + w.nfft ** 2 * w.n) * nact * frames,
-                                  "hbm_achieved_gbs": ach, "algorithmic_bytes": abytes}
+        result["config"]["path"] = "multi-WFS step (1 WFS): %s" % wb.info()[0][0]
+    if kavg:
+        result["roofline"] = roofline(args.form, w, nact, frames, kavg, abytes, noise)
+    oc = None
     if not args.no_cpu:
-        # CPU leg: the FP64 oracle on frame 0 of the same workload; its slopes also give the
-        # engine's error against FP64.
-        v, nthr, sample, oc = cpu_baseline(w, w.phase(0))
+        # CPU leg: the C restatement timed on this host; the FP64 oracle on frame 0 gives the
+        # error figure of every form.
+        v, nthr, sample, oc = cpu_baseline(w, w.phase(0), corr=corr)
         result["cpu_baseline"] = {"value": v, "unit": "subap-images/s", "cores": nthr, "kind": "port",
                                   "sample": sample}
-        if not noise and not w.correlation:
-            act = cc.subflag.ravel() == 1
-            d = (d_cents[0].cpu().numpy() - oc)[act]
-            result["rms_err_vs_fp64_px"] = float(np.sqrt((d ** 2).mean()))
-            result["max_err_vs_fp64_px"] = float(np.abs(d).max())
+    act = cc.subflag.ravel() == 1
+    if oc is not None and not noise:
+        d = (c0 - oc.reshape(-1, 2))[act]
+        result["rms_err_vs_fp64_px"] = float(np.sqrt((d ** 2).mean()))
+        result["max_err_vs_fp64_px"] = float(np.abs(d).max())
+    if args.compare_forms and wb is None and not noise and not w.correlation and w.n == 16 and w.nfft == 32 \
+            and w.nimg == 16:
+        # the headline compares the plain FP32 form with the MFMA DFT-GEMM forms on the same input
+        forms = {}
+        for form in ("fft32", "fp16", "bf16"):
+            if form == args.form:
+                fms, fk, fc = ms_step, kavg, c0
+            else:
+                fms, fk, fc = run_form(form)
+            e = {"value": nact * frames / (fms / 1e3), "ms_per_step": fms,
+                 "roofline": roofline(form, w, nact, frames, fk, abytes, noise) if fk else None}
+            if oc is not None:
+                d = (fc - oc)[act]
+                e["rms_err_vs_fp64_px"] = float(np.sqrt((d ** 2).mean()))
+                e["max_err_vs_fp64_px"] = float(np.abs(d).max())
+            forms[form] = e
+        cent.setForm(h, {"fft32": 0, "bf16": 1, "fp16": 2}[args.form])
+        result["forms"] = forms
     if wb is not None:
         wb.free()
     cc.free()
@@ -266,7 +319,7 @@ def bench_multi(args, dev):
             if z is not None:
                 z.copy_(torch.randn(z.shape, generator=gen, device=dev))
     torch.cuda.synchronize()
-    launches, others = wb.tune(reps=3)
+    launches, others = wb.tune(reps=3, cache=args.tune_cache)
     stream = torch.cuda.current_stream()
     for _ in range(args.warmup):
         wb.run()

@@ -98,10 +98,11 @@ def library():
                                                     ctypes.POINTER(ctypes.c_long), ctypes.POINTER(ip),
                                                     ctypes.POINTER(ctypes.c_float)]
         lib.dasp_cent_batch_others.argtypes = [vp]
+        lib.dasp_cent_batch_set_config.argtypes = [vp, ip, ip, ctypes.c_long, ip]
         lib.dasp_cent_batch_free.argtypes = [vp]
         for name in ("dasp_cent_batch_create", "dasp_cent_batch_tune", "dasp_cent_batch_run",
                      "dasp_cent_batch_launches", "dasp_cent_batch_launch_info", "dasp_cent_batch_others",
-                     "dasp_cent_batch_free"):
+                     "dasp_cent_batch_free", "dasp_cent_batch_set_config"):
             getattr(lib, name).restype = ctypes.c_int
         for name in ("dasp_cent_initialise", "dasp_cent_run", "dasp_cent_run_slope", "dasp_cent_update_int",
                      "dasp_cent_update_float", "dasp_cent_update_array", "dasp_cent_update_caldata",
@@ -387,19 +388,25 @@ def batchInfo(batch):
         _check(n)
     out = []
     for i in range(n):
-        shape = (ctypes.c_int * 4)()
+        shape = (ctypes.c_int * 5)()
         nwfs, waves, chunk = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
         groups, wgs = ctypes.c_long(), ctypes.c_long()
         ms = ctypes.c_float()
         _check(lib.dasp_cent_batch_launch_info(batch, i, shape, ctypes.byref(nwfs), ctypes.byref(groups),
                                                ctypes.byref(waves), ctypes.byref(wgs), ctypes.byref(chunk),
                                                ctypes.byref(ms)))
-        out.append(dict(np=shape[0], fftsize=shape[1], nimg=shape[2], spot=bool(shape[3]), nwfs=nwfs.value, groups=groups.value,
+        out.append(dict(np=shape[0], fftsize=shape[1], nimg=shape[2], spot=bool(shape[3]), fast=bool(shape[4]),
+                        nwfs=nwfs.value, groups=groups.value,
                         waves_per_wg=waves.value, workgroups=wgs.value, chunk=chunk.value, tuned_ms=ms.value))
     others = lib.dasp_cent_batch_others(batch)
     if others < 0:
         _check(others)
     return out, others
+
+
+def batchSetConfig(batch, i, waves_per_wg, workgroups, chunk=0):
+    """Apply a launch configuration (as reported by batchInfo) and rebuild the step graph."""
+    _check(library().dasp_cent_batch_set_config(batch, int(i), int(waves_per_wg), int(workgroups), int(chunk)))
 
 
 def batchFree(batch):

@@ -439,9 +439,21 @@ static bool fused_spot(dasp_cent *c) {
          c->d_otfk.p != nullptr;
 }
 
+// Centroid stage run by cent_corr.hip after the fused kernels: FFT correlation (mode 1) with a
+// fit or a centre-on-brightest CoG, or a fit on the image itself.
+static bool corr_stage(dasp_cent *c) {
+  const bool fit = c->parabolicFit != 0 || c->gaussianFit > 0;
+  return (c->correlationCentroiding == 1 && c->haveCorr) || (c->correlationCentroiding == 0 && fit);
+}
+
+static bool corr_stage_ok(dasp_cent *c) {
+  if (c->correlationCentroiding == 0) return !corr_stage(c) || corr_fit_supported(c->nimg);
+  return c->correlationCentroiding == 1 && c->haveCorr && corr_fit_supported(c->corrsize) && c->corrsize >= c->nimg;
+}
+
 static bool fused_options(dasp_cent *c) {
   return (c->spotDim == 0 || fused_spot(c)) && c->clip == c->F && c->atmosPhaseType == 0 && c->opticalBinning == 0 &&
-         c->imageOnly == 0 && c->correlationCentroiding == 0 && c->parabolicFit == 0 && c->gaussianFit == 0 &&
+         c->imageOnly == 0 && corr_stage_ok(c) &&
          !c->haveCalData && !c->haveCalCoeff && (c->calsource != 0 || (c->useBrightest == 0 && !c->haveUBArr));
 }
 
@@ -514,10 +526,42 @@ static int launch_phase(dasp_cent *c, const float *d_phs, long phs_stride, int n
   if (nitems > 0 && use_fused(c)) {
     FusedArgs fa;
     fused_args(c, d_phs, phs_stride, nframes, d_cents, d_bimg, d_zpois, d_zread, fa);
+    const bool stage = corr_stage(c);
+    if (stage && !d_bimg) return fail(DASP_CENT_EINVAL, "the correlation/fit stage needs the image buffer");
+    fa.no_cents = stage;
     int trc = timing_mark(c, st);
     if (trc) return trc;
     if (use_dft(c)) HIPCHK(launch_dft(fa, c->form == DASP_CENT_FORM_DFT_BF16, st));
     else HIPCHK(launch_fused(fa, np, c->F, c->spotDim != 0, st));
+    if (stage) {
+      CorrArgs ca{};
+      const bool corr = c->correlationCentroiding == 1;
+      const int N = corr ? c->corrsize : c->nimg;
+      ca.img = d_bimg;
+      ca.img_stride = (long)c->nsubaps * c->nimg * c->nimg;
+      ca.active = c->d_active.as<int>();
+      ca.nactive = (int)c->active.size();
+      ca.nitems = nitems;
+      ca.nimg = c->nimg;
+      ca.ncen = c->ncen;
+      ca.corrK = corr ? c->d_corrK.as<float2>() : nullptr;
+      ca.corrThresh = c->corrThresh;
+      ca.doFit = c->parabolicFit != 0 || c->gaussianFit > 0;
+      const int fs = c->parabolicFit + c->gaussianFit;
+      ca.fitsize = fs < 0 ? -1 : (fs < 3 ? 3 : fs);
+      ca.gaussianFit = c->gaussianFit;
+      ca.gaussianMinVal = c->gaussianMinVal;
+      ca.gaussianReplaceVal = c->gaussianReplaceVal;
+      std::memcpy(ca.fitMx, c->fitMx, sizeof(ca.fitMx));
+      ca.centWeight = c->haveCentWeight ? c->d_centWeight.as<float>() : nullptr;
+      ca.centWeightDim = c->haveCentWeight ? c->centWeightDim : 0;
+      ca.refCents = c->haveRef ? c->d_refCents.as<float>() : nullptr;
+      ca.cents = d_cents;
+      ca.cents_stride = (long)c->nsubaps * 2;
+      ca.corrimg = corr ? d_corrimg : nullptr;
+      ca.corr_stride = (long)c->nsubaps * N * N;
+      HIPCHK(launch_corr_fit(ca, N, corr, st));
+    }
     trc = timing_mark(c, st);
     if (trc) return trc;
   } else if (nitems > 0) {
@@ -644,7 +688,7 @@ static int check_device_run(dasp_cent *c, const float *d_phs, long phs_stride, i
 
 // Images and correlation images the general path materialises even when the caller wants none.
 static int device_scratch(dasp_cent *c, int nframes, float **bimg, float **corr) {
-  if (!*bimg && !use_fused(c)) {
+  if (!*bimg && (!use_fused(c) || corr_stage(c))) {
     HIPCHK(c->d_bscratch.ensure((size_t)nframes * c->nsubaps * c->nimg * c->nimg * 4));
     *bimg = c->d_bscratch.as<float>();
   }
@@ -693,7 +737,8 @@ struct dasp_cent_batch {
 };
 
 static bool batch_fusable(dasp_cent *c) {
-  return use_fused(c) && !use_dft(c) && batch_supported(pad_pow2(c->n), c->F, c->nimg, c->spotDim != 0);
+  return use_fused(c) && !use_dft(c) && !corr_stage(c) &&
+         batch_supported(pad_pow2(c->n), c->F, c->nimg, c->spotDim != 0);
 }
 
 static void batch_default_config(dasp_cent_batch *b, BatchLaunch &L) {
@@ -1481,7 +1526,22 @@ int dasp_cent_batch_launches(dasp_cent_batch_t *b) {
   return (int)b->launches.size();
 }
 
-int dasp_cent_batch_launch_info(dasp_cent_batch_t *b, int i, int shape[4], int *nwfs, long *groups,
+int dasp_cent_batch_set_config(dasp_cent_batch_t *b, int i, int waves_per_wg, long workgroups, int chunk) {
+  if (!b) return fail(DASP_CENT_EINVAL, "null batch");
+  int rc = batch_refresh(b);
+  if (rc) return rc;
+  if (i < 0 || i >= (int)b->launches.size()) return fail(DASP_CENT_EINVAL, "launch %d out of range", i);
+  if (waves_per_wg != 1 && waves_per_wg != 2 && waves_per_wg != 4)
+    return fail(DASP_CENT_EINVAL, "waves per workgroup must be 1, 2 or 4");
+  if (workgroups < 1 || workgroups > 0x7fffffffL || chunk < 0) return fail(DASP_CENT_EINVAL, "bad workgroups/chunk");
+  BatchLaunch &L = b->launches[i];
+  L.waves = waves_per_wg;
+  L.wgs = workgroups;
+  L.chunk = chunk;
+  return batch_capture(b);
+}
+
+int dasp_cent_batch_launch_info(dasp_cent_batch_t *b, int i, int shape[5], int *nwfs, long *groups,
                                 int *waves_per_wg, long *workgroups, int *chunk, float *ms) {
   if (!b) return fail(DASP_CENT_EINVAL, "null batch");
   if (i < 0 || i >= (int)b->launches.size()) return fail(DASP_CENT_EINVAL, "launch %d out of range", i);
@@ -1491,6 +1551,7 @@ int dasp_cent_batch_launch_info(dasp_cent_batch_t *b, int i, int shape[4], int *
     shape[1] = L.F;
     shape[2] = L.nimg;
     shape[3] = L.spot;
+    shape[4] = L.fast;
   }
   if (nwfs) *nwfs = (int)L.items.size();
   if (groups) *groups = L.groups;

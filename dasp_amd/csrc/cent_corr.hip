@@ -1,0 +1,285 @@
+// cent_corr.hip - centroid stage after the fused kernels for correlation centroiding and fits:
+// calcCorrelation (FFT, mode 1) + thresholdCorrelation, then computeFit (parabolic, Gaussian
+// or 3x3 quadratic interpolation) or computeCoG centred on the brightest pixel, then refCents
+// (cmod/src/centmodule.c:1403-1505, :1069-1188, :873-971, :1899-1961).
+//
+// The fused image kernel writes each item's final detector image (after noise, threshold and
+// pixel power); this kernel reads it back. One group of N lanes per item (N = corrsize, or nimg
+// when there is no correlation), 64/N items per wave: lane r holds row r of the centre-padded
+// image, transforms it in registers, the wave's LDS carries the two transposes, lane r then owns
+// spectrum column r for the product with the pattern spectrum K (calcCorrelation(img) =
+// IDFT2(DFT2(pad(img)) K), unnormalised, K built on the host from the half-complex pattern).
+// The fit's window sums run in one lane in the order of the general kernel.
+#include "cent_internal.h"
+#include "fft_reg.h"
+
+namespace dasp {
+
+namespace {
+
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template <int W>
+__device__ __forceinline__ float gsum(float v) {
+  static_for<0, 6>([&](auto b) {
+    constexpr int m = 1 << decltype(b)::value;
+    if constexpr (m < W) v += __shfl_xor(v, m);
+  });
+  return v;
+}
+
+template <int W>
+__device__ __forceinline__ float gmax(float v) {
+  static_for<0, 6>([&](auto b) {
+    constexpr int m = 1 << decltype(b)::value;
+    if constexpr (m < W) v = fmaxf(v, __shfl_xor(v, m));
+  });
+  return v;
+}
+
+// (value, index) maximum with the first index winning ties; value 0 / index 0 when nothing is
+// positive (the reference's scan starts from mx = 0 with a strict comparison).
+template <int W>
+__device__ __forceinline__ void gargmax(float &v, int &i) {
+  static_for<0, 6>([&](auto b) {
+    constexpr int m = 1 << decltype(b)::value;
+    if constexpr (m < W) {
+      const float ov = __shfl_xor(v, m);
+      const int oi = __shfl_xor(i, m);
+      if (ov > v || (ov == v && oi < i)) {
+        v = ov;
+        i = oi;
+      }
+    }
+  });
+}
+
+}  // namespace
+
+template <int N, bool CORR>
+__global__ __launch_bounds__(256) void corr_fit_kernel(CorrArgs a) {
+  constexpr int IPW = 64 / N, TS = N + 1, WAVES = 4;
+  __shared__ cf stage[WAVES][IPW * N * TS];
+  const int lane = threadIdx.x & 63;
+  const int w = threadIdx.x >> 6;
+  const int sl = lane / N;
+  const int r = lane - sl * N;
+  cf *T = &stage[w][sl * N * TS];
+  float *Tf = reinterpret_cast<float *>(T);
+  const long ngroups = (a.nitems + IPW - 1) / IPW;
+  for (long g = (long)blockIdx.x * WAVES + w; g < ngroups; g += (long)gridDim.x * WAVES) {
+    const long item = g * IPW + sl;
+    const bool valid = item < a.nitems;
+    int frame = 0, s = 0;
+    if (valid) {
+      frame = (int)(item / a.nactive);
+      s = a.active[item - (long)frame * a.nactive];
+    }
+    const int M = a.nimg;
+    const float *img = a.img + (long)frame * a.img_stride + (long)s * M * M;
+    float row[N];
+    if constexpr (CORR) {
+      // centre-pad to N, row r of the padded image
+      const int o = (N - M) / 2;
+      const int ir = r - o;
+      cf x[N];
+      static_for<0, N>([&](auto cc) {
+        constexpr int c = decltype(cc)::value;
+        const bool in = valid && ir >= 0 && ir < M && c - o >= 0 && c - o < M;
+        x[c] = {in ? img[ir * M + c - o] : 0.f, 0.f};
+      });
+      fft_fwd<N>(x);
+      static_for<0, N>([&](auto v) { T[r * TS + decltype(v)::value] = x[decltype(v)::value]; });
+      wave_sync();
+      cf y[N];
+      static_for<0, N>([&](auto u) { y[decltype(u)::value] = T[decltype(u)::value * TS + r]; });
+      wave_sync();
+      fft_fwd<N>(y);
+      const float2 *kp = a.corrK + (long)s * N * N + r;
+      static_for<0, N>([&](auto u) {
+        const float2 k = kp[decltype(u)::value * N];
+        y[decltype(u)::value] = cmul(y[decltype(u)::value], {k.x, k.y});
+      });
+      fft_inv<N>(y);
+      static_for<0, N>([&](auto u) { T[decltype(u)::value * TS + r] = y[decltype(u)::value]; });
+      wave_sync();
+      static_for<0, N>([&](auto v) { x[decltype(v)::value] = T[r * TS + decltype(v)::value]; });
+      wave_sync();
+      fft_inv<N>(x);
+      static_for<0, N>([&](auto c) { row[decltype(c)::value] = x[decltype(c)::value].x; });
+      // thresholdCorrelation (:1488-1505)
+      float t = a.corrThresh;
+      if (t <= 1.f && t > 0.f) {
+        float mx = 0.f;
+        static_for<0, N>([&](auto c) { mx = fmaxf(mx, row[decltype(c)::value]); });
+        t = gmax<N>(mx) * t;
+      }
+      static_for<0, N>([&](auto c) {
+        const float v = row[decltype(c)::value];
+        row[decltype(c)::value] = v < t ? 0.f : v - t;
+      });
+    } else {
+      static_for<0, N>([&](auto c) { row[decltype(c)::value] = valid ? img[r * N + decltype(c)::value] : 0.f; });
+    }
+
+    // sum and first maximum over the whole image (computeFit :1087-1120; computeCoG :941-969)
+    float sum = 0.f, mv = 0.f;
+    int mi = 0x7fffffff;
+    static_for<0, N>([&](auto cc) {
+      constexpr int c = decltype(cc)::value;
+      sum += row[c];
+      if (row[c] > mv) {
+        mv = row[c];
+        mi = r * N + c;
+      }
+    });
+    sum = gsum<N>(sum);
+    gargmax<N>(mv, mi);
+    const int mxpos = mv > 0.f ? mi : 0;
+    float cx = 0.f, cy = 0.f;
+    const int ncen = a.ncen;
+    if (a.doFit && a.gaussianFit) {
+      // computeFit takes the logarithm in place (:1087-1098) on the buffer it works on: the
+      // correlation image when correlating, otherwise the detector image; both are returned
+      // transformed
+      static_for<0, N>([&](auto c) {
+        const float v = row[decltype(c)::value];
+        row[decltype(c)::value] = v < a.gaussianMinVal ? a.gaussianReplaceVal : logf(v);
+      });
+      if (!CORR && valid) {
+        float *io = const_cast<float *>(img) + r * N;
+        static_for<0, N>([&](auto c) { io[decltype(c)::value] = row[decltype(c)::value]; });
+      }
+    }
+    if (CORR && a.corrimg && valid) {
+      float *co = a.corrimg + (long)frame * a.corr_stride + (long)s * N * N + r * N;
+      static_for<0, N>([&](auto c) { co[decltype(c)::value] = row[decltype(c)::value]; });
+    }
+    if (a.doFit) {
+      static_for<0, N>([&](auto c) { Tf[r * N + decltype(c)::value] = row[decltype(c)::value]; });
+      wave_sync();
+      if (r == 0 && sum >= 0.f) {
+        const int nc = (N - ncen) / 2;
+        int fitsize = a.fitsize;
+        const bool quad = fitsize < 0;
+        if (quad || fitsize < 3) fitsize = 3;
+        int startx = mxpos % N - fitsize / 2;
+        int starty = mxpos / N - fitsize / 2;
+        if (startx < nc) startx = nc;
+        if (starty < nc) starty = nc;
+        if (startx > nc + ncen - fitsize) startx = nc + ncen - fitsize;
+        if (starty > nc + ncen - fitsize) starty = nc + ncen - fitsize;
+        float x, y;
+        if (!quad) {
+          float vec[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+#pragma unroll 1
+          for (int yy = 0; yy < fitsize; ++yy)
+#pragma unroll 1
+            for (int xx = 0; xx < fitsize; ++xx) {
+              const float val = Tf[(yy + starty) * N + xx + startx];
+              vec[0] += val * xx * xx;
+              vec[1] += val * xx * yy;
+              vec[2] += val * yy * yy;
+              vec[3] += val * xx;
+              vec[4] += val * yy;
+              vec[5] += val;
+            }
+          float res[5];
+          for (int i = 0; i < 5; ++i) {
+            float t = 0.f;
+            for (int j = 0; j < 6; ++j) t += a.fitMx[i * 6 + j] * vec[j];
+            res[i] = t;
+          }
+          if (a.gaussianFit) {
+            x = -res[3] / (2 * res[0]);
+            y = -res[4] / (2 * res[2]);
+          } else {
+            x = (res[1] * res[4] / (2 * res[2]) - res[3]) / (2.f * res[0] - res[1] * res[1] / (2.f * res[2]));
+            y = -(res[4] + res[1] * x) / (2.f * res[2]);
+          }
+        } else {
+          const float *q = &Tf[starty * N + startx];
+          const float a2 = (q[2 * N + 1] - q[1]) / 2.f;
+          const float a3 = (q[2 * N + 1] - 2 * q[N + 1] + q[1]) / 2.f;
+          const float a4 = (q[N + 2] - q[N]) / 2.f;
+          const float a5 = (q[N + 2] - 2 * q[N + 1] + q[N]) / 2.f;
+          const float a6 = (q[2 * N + 2] - q[2] - q[2 * N] + q[0]) / 4.f;
+          y = (2 * a2 * a5 - a4 * a6) / (a6 * a6 - 4 * a3 * a5) + 1;
+          x = (2 * a3 * a4 - a2 * a6) / (a6 * a6 - 4 * a3 * a5) + 1;
+        }
+        if (x < 0 || y < 0 || x >= fitsize || y >= fitsize) {
+          x = (float)(mxpos % N - startx);
+          y = (float)(mxpos / N - starty);
+        }
+        y -= ncen / 2.f - 0.5f;
+        x -= ncen / 2.f - 0.5f;
+        x += startx + ncen / 2.f - N / 2.f;
+        y += starty + ncen / 2.f - N / 2.f;
+        cx = x;
+        cy = y;
+      }
+      wave_sync();
+    } else {
+      // computeCoG centred on the brightest pixel (corr mode 1), optionally weighted
+      int sx = (int)truncf(mxpos % N - ncen / 2.f + 0.5f);
+      int sy = (int)truncf(mxpos / N - ncen / 2.f + 0.5f);
+      sx = min(max(sx, 0), N - ncen);
+      sy = min(max(sy, 0), N - ncen);
+      const float *wt = a.centWeight ? a.centWeight + (a.centWeightDim == 4 ? (long)s * N * N : 0L) : nullptr;
+      float sw = 0.f, sxw = 0.f, syw = 0.f;
+      if (r >= sy && r < sy + ncen) {
+        static_for<0, N>([&](auto cc) {
+          constexpr int c = decltype(cc)::value;
+          if (c >= sx && c < sx + ncen) {
+            const float v = wt ? row[c] * wt[r * N + c] : row[c];
+            sw += v;
+            sxw += v * c;
+            syw += v * r;
+          }
+        });
+      }
+      sw = gsum<N>(sw);
+      sxw = gsum<N>(sxw);
+      syw = gsum<N>(syw);
+      if (sw > 0.f) {
+        cx = sxw / sw + sx - (N - 1) / 2.f;
+        cy = syw / sw + sy - (N - 1) / 2.f;
+      }
+    }
+    if (valid && r == 0) {
+      if (a.refCents) {
+        cx -= a.refCents[s * 2];
+        cy -= a.refCents[s * 2 + 1];
+      }
+      float *co = a.cents + (long)frame * a.cents_stride + s * 2;
+      co[0] = cx;
+      co[1] = cy;
+    }
+  }
+}
+
+bool corr_fit_supported(int n) { return n == 8 || n == 16 || n == 32; }
+
+hipError_t launch_corr_fit(const CorrArgs &a, int n, bool corr, hipStream_t st) {
+  const long ipb = 4L * (64 / n);
+  long blocks = (a.nitems + ipb - 1) / ipb;
+  if (blocks <= 0) return hipSuccess;
+  if (blocks > 2048) blocks = 2048;
+#define DASP_CORR_CASE(NN)                                                                                  \
+  if (n == NN) {                                                                                          \
+    if (corr) hipLaunchKernelGGL((corr_fit_kernel<NN, true>), dim3((unsigned)blocks), dim3(256), 0, st, a);  \
+    else hipLaunchKernelGGL((corr_fit_kernel<NN, false>), dim3((unsigned)blocks), dim3(256), 0, st, a);      \
+    return hipGetLastError();                                                                            \
+  }
+  DASP_CORR_CASE(8)
+  DASP_CORR_CASE(16)
+  DASP_CORR_CASE(32)
+#undef DASP_CORR_CASE
+  return hipErrorInvalidValue;
+}
+
+}  // namespace dasp

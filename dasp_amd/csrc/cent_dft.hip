@@ -322,7 +322,7 @@ __global__ __launch_bounds__(256) void dft16_cog_kernel(FusedArgs a) {
       sw = wave_sum(sw);
       sxw = wave_sum(sxw);
       syw = wave_sum(syw);
-      if (l == 0) {
+      if (l == 0 && !a.no_cents) {
         float cx = 0.f, cy = 0.f;
         if (sw > 0.f) {
           cx = sxw / sw - (kM - 1) / 2.f;

@@ -13,8 +13,14 @@
 
 namespace dasp {
 
-// Persistent waves: wave w handles item groups w, w + nwaves, ... and loads the next group's
-// phase and pupil while it transforms the current one.
+// Persistent waves over groups of SPW consecutive active subapertures of one frame. XCD-aware:
+// workgroup b runs on XCD b % 8 (round-robin dispatch), and XCD x owns the x-th eighth of the
+// subaperture groups for every frame, so each XCD's L2 keeps its slice of the pupil (and of any
+// per-subaperture table) across frames instead of every frame re-reading the whole pupil. Inside
+// an XCD the groups are taken frame-major with a grid stride, and each wave loads its next
+// group's phase and pupil while it transforms the current one.
+constexpr int kXcds = 8;
+
 template <int NP, int F, int B, bool FAST, bool SPOT>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NP <= 16 ? 2 : 1)))
 void fused_cog_kernel(FusedArgs a) {
@@ -25,17 +31,26 @@ void fused_cog_kernel(FusedArgs a) {
   const int w = threadIdx.x >> 6;
   const int sl = lane / NP;
   const int j = lane - sl * NP;   // pass 1: phase column; pass 2: row group m
-  const long ngroups = (a.nitems + SPW - 1) / SPW;
-  const long stride = (long)gridDim.x * S::WAVES;
-  long g = (long)blockIdx.x * S::WAVES + w;
-  if (g >= ngroups) return;       // wave-uniform
+  const int xcd = blockIdx.x % kXcds;
+  const long nsg = (a.nactive + SPW - 1) / SPW;              // subaperture groups per frame
+  const long sg0 = nsg * xcd / kXcds, nsgx = nsg * (xcd + 1) / kXcds - sg0;
+  const long nframes = a.nitems / a.nactive;
+  const long ngx = nsgx * nframes;                           // this XCD's groups
+  const long stride = (long)(gridDim.x / kXcds) * S::WAVES;
+  long q = (long)(blockIdx.x / kXcds) * S::WAVES + w;
+  if (q >= ngx) return;           // wave-uniform
+  auto item_of = [&](long qq) -> long {
+    const long f = qq / nsgx;
+    const long idx = (sg0 + qq - f * nsgx) * SPW + sl;
+    return qq < ngx && idx < a.nactive ? f * a.nactive + idx : -1;
+  };
   cf *st = &stage[w][sl * SUB];
   PhaseCol<NP> cur;
-  load_item<NP>(a, g * SPW + sl, j, cur);
-  for (; g < ngroups; g += stride) {
+  load_item<NP>(a, item_of(q), j, cur);
+  for (; q < ngx; q += stride) {
     PhaseCol<NP> nxt;
-    load_item<NP>(a, (g + stride) * SPW + sl, j, nxt);   // items past the end load zeros
-    fused_item<NP, F, B, FAST, SPOT>(a, g * SPW + sl, j, w, sl, cur, st);
+    load_item<NP>(a, item_of(q + stride), j, nxt);   // past the end: zeros
+    fused_item<NP, F, B, FAST, SPOT>(a, item_of(q), j, w, sl, cur, st);
     cur = nxt;
   }
 }
@@ -43,12 +58,14 @@ void fused_cog_kernel(FusedArgs a) {
 template <int NP, int F, int B, bool SPOT>
 static hipError_t launch_fused_t(const FusedArgs &a, hipStream_t st) {
   using S = FusedShape<NP, F, B>;
-  const long per_block = (long)S::WAVES * S::SPW;
-  long blocks = (a.nitems + per_block - 1) / per_block;
-  if (blocks <= 0) return hipSuccess;
-  // persistent grid: enough workgroups for every CU to hold several, then grid-stride
+  if (a.nitems <= 0 || a.nactive <= 0) return hipSuccess;
+  const long nsg = (a.nactive + S::SPW - 1) / S::SPW;
+  const long groups = nsg * (a.nitems / a.nactive);
+  long blocks = (groups + S::WAVES - 1) / S::WAVES;
+  // persistent grid: enough workgroups for every CU to hold several, a multiple of the XCD count
   const long cap = 256L * 4;
   if (blocks > cap) blocks = cap;
+  blocks = (blocks + kXcds - 1) / kXcds * kXcds;
   if (a.fast_trig)
     hipLaunchKernelGGL((fused_cog_kernel<NP, F, B, true, SPOT>), dim3((unsigned)blocks), dim3(64 * S::WAVES), 0, st, a);
   else

@@ -144,7 +144,32 @@ struct FusedArgs {
   // (F/2 + 1) rows of F complex, see spot_convolve in fused_item.h
   const float2 *otfk;
   int otf_per_subap;
+  int no_cents;          // slopes come from the correlation/fit stage (cent_corr.hip)
 };
+
+// Correlation / fit stage after the fused kernels (cent_corr.hip).
+struct CorrArgs {
+  const float *img;      // final detector images (frames, nsubaps, nimg, nimg)
+  long img_stride;
+  const int *active;
+  int nactive;
+  long nitems;
+  int nimg, ncen;
+  const float2 *corrK;   // (nsubaps, N, N) pattern spectrum, mode 1
+  float corrThresh;
+  int doFit, fitsize, gaussianFit;
+  float gaussianMinVal, gaussianReplaceVal;
+  float fitMx[30];
+  const float *centWeight;
+  int centWeightDim;
+  const float *refCents;
+  float *cents;
+  long cents_stride;
+  float *corrimg;
+  long corr_stride;
+};
+bool corr_fit_supported(int n);
+hipError_t launch_corr_fit(const CorrArgs &a, int n, bool corr, hipStream_t st);
 
 // Multi-WFS persistent step (batch_kernel.h): one descriptor per WFS in a shape's launch.
 struct BatchDesc {

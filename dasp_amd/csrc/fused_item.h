@@ -76,7 +76,7 @@ struct PhaseCol {
 // Load phase column j (rows i < n) and the pupil of one work item; zeros where out of range.
 template <int NP>
 __device__ __forceinline__ void load_item(const FusedArgs &a, long item, int j, PhaseCol<NP> &c) {
-  const bool valid = item < a.nitems;
+  const bool valid = item >= 0 && item < a.nitems;   // item < 0: padding lane
   int frame = 0, s = 0;
   if (valid) {
     frame = (int)(item / a.nactive);
@@ -195,7 +195,7 @@ __device__ __forceinline__ void fused_item(const FusedArgs &a, long item, int j,
                                            const PhaseCol<NP> &col, cf *st) {
   using S = FusedShape<NP, F, B>;
   constexpr int R = S::R, NIMG = S::NIMG, RB = S::RB, G = S::G, TS = S::TS;
-  const bool valid = item < a.nitems;
+  const bool valid = item >= 0 && item < a.nitems;   // item < 0: padding lane
   int frame = 0, s = 0;
   if (valid) {
     frame = (int)(item / a.nactive);
@@ -352,7 +352,7 @@ __device__ __forceinline__ void fused_item(const FusedArgs &a, long item, int j,
   sw = group_sum<NP>(sw);
   sxw = group_sum<NP>(sxw);
   syw = group_sum<NP>(syw);
-  if (valid && j == 0) {
+  if (valid && j == 0 && !a.no_cents) {
     float cx = 0.f, cy = 0.f;
     if (sw > 0.f) {
       cx = sxw / sw - (NIMG - 1) / 2.f;

@@ -63,9 +63,36 @@ class WFSBatch:
     def stream(self):
         return self.torch.cuda.current_stream(self.dev).cuda_stream
 
-    def tune(self, reps=5):
+    @staticmethod
+    def _key(L):
+        return "np%d_F%d_nimg%d_spot%d_fast%d_groups%d" % (L["np"], L["fftsize"], L["nimg"], L["spot"], L["fast"],
+                                                          L["groups"])
+
+    def tune(self, reps=5, cache=None):
+        """Autotune every persistent launch. With cache (a JSON file path), launches whose key
+        (shape, trig mode, queue length) is in the file take the stored configuration instead,
+        and newly tuned ones are added to it."""
+        import json
+        import os
+        stored = {}
+        if cache and os.path.exists(cache):
+            with open(cache) as f:
+                stored = json.load(f)
+        launches, _ = self.info()
+        if stored and all(self._key(L) in stored for L in launches):
+            for i, L in enumerate(launches):
+                c = stored[self._key(L)]
+                cent.batchSetConfig(self.batch, i, c["waves_per_wg"], c["workgroups"], c["chunk"])
+            return self.info()
         cent.batchTune(self.batch, self.stream(), reps)
-        return self.info()
+        launches, others = self.info()
+        if cache:
+            for L in launches:
+                stored[self._key(L)] = dict(waves_per_wg=L["waves_per_wg"], workgroups=L["workgroups"],
+                                            chunk=L["chunk"], tuned_ms=L["tuned_ms"])
+            with open(cache, "w") as f:
+                json.dump(stored, f, indent=1)
+        return launches, others
 
     def run(self):
         cent.batchRun(self.batch, self.stream())

@@ -225,12 +225,16 @@ int dasp_cent_batch_tune(dasp_cent_batch_t *b, void *stream, int reps);
 int dasp_cent_batch_run(dasp_cent_batch_t *b, void *stream);
 
 /* Number of persistent launches in the step, and the configuration of launch i: its shape
- * (phase pad NP, fftsize, nimg, 1 if the WFS convolve with a spot kernel), number of WFS, queue length in groups of 64/NP work items,
+ * (phase pad NP, fftsize, nimg, 1 if the WFS convolve with a spot kernel, 1 for fast trig), number of WFS, queue length in groups of 64/NP work items,
  * waves per workgroup, workgroups, schedule (0 static, c > 0 dynamic claims of c groups) and
  * the tuned time in ms (-1 before tuning). */
 int dasp_cent_batch_launches(dasp_cent_batch_t *b);
-int dasp_cent_batch_launch_info(dasp_cent_batch_t *b, int i, int shape[4], int *nwfs, long *groups,
+int dasp_cent_batch_launch_info(dasp_cent_batch_t *b, int i, int shape[5], int *nwfs, long *groups,
                                 int *waves_per_wg, long *workgroups, int *chunk, float *ms);
+
+/* Set the configuration of launch i (e.g. one the autotuner chose in an earlier process) and
+ * rebuild the step graph. */
+int dasp_cent_batch_set_config(dasp_cent_batch_t *b, int i, int waves_per_wg, long workgroups, int chunk);
 
 /* Number of WFS that run outside the persistent launches (their own kernels in the graph). */
 int dasp_cent_batch_others(dasp_cent_batch_t *b);

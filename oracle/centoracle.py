@@ -28,6 +28,8 @@ is therefore checked against mathematics or an independent route:
   transform_psf          its half-complex unpack equals conj(fft2(fftshift(psf))).
   make_fit_mx/compute_fit exact vertex recovery for sampled paraboloids; fit matrix equals the
                          inverse of the normal matrix from an independent construction.
+  gaussian_log           exact centre of a sampled 2-D Gaussian through the Gaussian fit; the
+                         returned image of a Gaussian-fit run equals the log of a plain run's.
   poisson_from_normal    exact inverse-CDF below lambda=10 (checked against scipy-free pmf sums);
                          mean/variance and total-variation distance to the exact pmf above it.
   diff_correlation, brute_correlation
@@ -733,6 +735,14 @@ def make_fit_mx(fitsize):
     return np.linalg.inv(mx)[:5]
 
 
+def gaussian_log(img, gmin, grep):
+    """The in-place transform of computeFit's Gaussian branch, cmod/src/centmodule.c:1087-1098
+    (logf in FP32 there)."""
+    x = np.asarray(img, f64)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        return np.where(x < gmin, grep, np.log(np.where(x < gmin, 1.0, x)))
+
+
 def compute_fit(img, nimg, ncen, fitsize, do_gaussian=0, gmin=0.0, grep=0.0, corr_mode=0):
     """computeFit, cmod/src/centmodule.c:1069-1188 on one (nimg,nimg) image (nimg is the
     reference's corrsize). Returns (cx, cy)."""
@@ -970,6 +980,14 @@ def _image_backend(p, img, s, nexposed_full):
     elif p.parabolicFit != 0 or p.gaussianFit > 0:
         cx, cy = compute_fit(src, size, p.ncen, p.parabolicFit + p.gaussianFit, p.gaussianFit,
                              p.gaussianMinVal, p.gaussianReplaceVal, cc)
+        if p.gaussianFit:
+            # computeFit takes the logarithm in place (:1087-1098) on the buffer it is given: the
+            # correlation image when correlating (:1899-1906), otherwise the output image (:1919)
+            g = gaussian_log(src, p.gaussianMinVal, p.gaussianReplaceVal)
+            if corr_out is not None:
+                corr_out = g
+            else:
+                out_img = g
     else:
         w = None
         if p.centWeight is not None:

@@ -154,6 +154,10 @@ OPTION_CASES = {
     "diffcorr4": "diff4",
     "gaussfit": "gauss",
     "quadinterp": "quad",
+    "corr1_cog": "corrcog",
+    "corr1_fit3_thresh": "corrthresh",
+    "parabolic5_image": dict(parabolicFit=5),
+    "gauss_image": dict(gaussianFitVals=(1e-4, -12.)),
 }
 
 
@@ -173,14 +177,16 @@ def test_options(name, path):
         kw = dict(skybrightness=(rng.random((S, nimg, nimg)) * 0.01).astype(np.float32))
     elif kw == "centweight":
         kw = dict(centWeight=(rng.random((nsubx, nsubx, nimg, nimg)) + 0.5).astype(np.float32))
-    elif kw in ("brute", "diff2", "diff4", "gauss", "quad"):
+    elif kw in ("brute", "diff2", "diff4", "gauss", "quad", "corrcog", "corrthresh"):
         ref = W.reference_images(wgeo)
-        mode = {"brute": 5, "diff2": 2, "diff4": 4, "gauss": 1, "quad": 1}[kw]
+        mode = {"brute": 5, "diff2": 2, "diff4": 4, "gauss": 1, "quad": 1, "corrcog": 1, "corrthresh": 1}[kw]
         kw = dict(correlationCentroiding=mode, corrPattern=ref.astype(np.float32))
         if name == "gaussfit":
             kw.update(gaussianFitVals=(1e-3, -10.))
         if name == "quadinterp":
             kw.update(parabolicFit=-1)
+        if name == "corr1_fit3_thresh":
+            kw.update(parabolicFit=3, corrThresh=0.3)
     kw = dict(kw)
     nimg_ = kw.pop("nimg", nimg)
     c = Case(nsubx, n, nfft, nimg_, obs=0.25, phs=phs, form=PATHS[path], **kw)
@@ -191,6 +197,10 @@ def test_options(name, path):
     if not c.params.imageOnly:
         d = report(name, c.cents, oc, act)
         assert d.max() < 1e-4
+    if ocorr is not None and c.corrimg is not None:
+        C = c.corrimg.shape[-1]
+        ci = c.corrimg.reshape(-1, C, C)[act]
+        np.testing.assert_allclose(ci, ocorr[act], rtol=0, atol=1e-4 * max(1e-30, np.abs(ocorr[act]).max()))
     c.free()
 
 

@@ -336,3 +336,29 @@ def test_brute_and_diff_correlation():
     assert d[K // 2 + o, K // 2 + o] == 0.0
     nz = d[o:o + K, o:o + K]
     assert (nz[nz != 0] > 0).all()
+
+
+def test_gaussian_fit_vertex_and_in_place_log():
+    """computeFit's Gaussian branch fits the parabola to log(img): a sampled 2-D Gaussian is
+    recovered exactly; and the transform is applied to the output buffer (cmod/src/centmodule.c
+    :1087-1098 on bimg, :1919), so the returned image is the logarithm of the plain one."""
+    N, K = 16, 12
+    y, x = np.mgrid[0:N, 0:N].astype(float)
+    x0, y0 = 7.35, 8.1
+    img = 50 * np.exp(-((x - x0) ** 2 / 6.0 + (y - y0) ** 2 / 4.0))
+    cx, cy = O.compute_fit(img, N, K, 5, do_gaussian=1, gmin=1e-9, grep=-30.)
+    assert abs(cx - (x0 + 0.5 - N / 2)) < 1e-9 and abs(cy - (y0 + 0.5 - N / 2)) < 1e-9
+    nsubx, n, nfft, nimg = 2, 8, 16, 8
+    pup = obstructed_pupil(nsubx, n, 0.0, 0.5)
+    pupfn = pup.perSubap(nsubx, vectorAlign=0)
+    flag = pup.getSubapFlag(nsubx, 0.5)
+    frac = (pup.getSubarea(nsubx) / (n * n)).astype(np.float32)
+    phs = np.random.default_rng(4).normal(size=(nsubx * n, nsubx * n)) * 0.3
+    plain = O.Params(nsubx, n, nfft, nfft, nimg, nimg, pupfn, flag, frac, sig=100.0)
+    gauss = O.Params(nsubx, n, nfft, nfft, nimg, nimg, pupfn, flag, frac, sig=100.0, gaussianFit=1,
+                     gaussianMinVal=1e-3, gaussianReplaceVal=-12.0)
+    _, b0, _ = O.run_phase(plain, phs)
+    _, b1, _ = O.run_phase(gauss, phs)
+    act = flag.ravel() == 1
+    want = np.where(b0 < 1e-3, -12.0, np.log(np.maximum(b0, 1e-300)))
+    np.testing.assert_allclose(b1[act], want[act], rtol=1e-12, atol=1e-12)
