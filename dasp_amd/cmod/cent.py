@@ -15,7 +15,9 @@ import os
 
 import numpy
 
-_LIBPATH = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "libdasp_cent.so")
+# DASP_CENT_LIB: path of another build of the library (default: the in-tree build)
+_LIBPATH = os.environ.get("DASP_CENT_LIB") or os.path.join(
+    os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "libdasp_cent.so")
 
 
 class error(Exception):

@@ -15,12 +15,6 @@
 
 namespace dasp {
 
-__device__ __forceinline__ long wave_uniform(unsigned long long v) {
-  const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)v);
-  const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(v >> 32));
-  return (long)(((unsigned long long)hi << 32) | lo);
-}
-
 // Work schedule of one wave over the queue. chunk == 0: static, wave w of W takes entries
 // w, w + W, w + 2W, ... (no atomics; balanced when entries cost the same, as they do within one
 // shape). chunk > 0: dynamic, the wave claims chunk consecutive entries per atomic on the
@@ -77,7 +71,7 @@ void batch_kernel(BatchArgs b, int chunk) {
   constexpr int SPW = 64 / NP, SUB = stage_per_subap<NP, F, SPOT>();
   __shared__ cf stage[WAVES][SPW * SUB];
   const int lane = threadIdx.x & 63;
-  const int w = threadIdx.x >> 6;
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int sl = lane / NP;
   const int j = lane - sl * NP;
   cf *st = &stage[w][sl * SUB];
@@ -87,16 +81,22 @@ void batch_kernel(BatchArgs b, int chunk) {
     int d = desc_of(b, g);
     long base = b.desc[d].group0, end = desc_end(b, d);
     PhaseCol<NP> cur;
-    load_item<NP>(desc_args(b, d), (g - base) * SPW + sl, j, cur);
+    ItemRef it = item_flat(desc_args(b, d), (g - base) * SPW, sl);
+    load_item<NP>(desc_args(b, d), it, j, cur);
     while (true) {
       const long gn = q.next(g);   // entries only grow: gn > g
       const bool more = gn < b.total_groups;
       const int dn = more && gn >= end ? desc_of(b, gn) : d;
       PhaseCol<NP> nxt;
-      if (more) load_item<NP>(desc_args(b, dn), (gn - b.desc[dn].group0) * SPW + sl, j, nxt);
-      fused_item<NP, F, B, FAST, SPOT>(desc_args(b, d), (g - base) * SPW + sl, j, w, sl, cur, st);
+      ItemRef itn{0, 0, false};
+      if (more) {
+        itn = item_flat(desc_args(b, dn), (gn - b.desc[dn].group0) * SPW, sl);
+        load_item<NP>(desc_args(b, dn), itn, j, nxt);
+      }
+      fused_item<NP, F, B, FAST, SPOT>(desc_args(b, d), it, j, w, sl, cur, st);
       if (!more) break;
       cur = nxt;
+      it = itn;
       g = gn;
       if (dn != d) {
         d = dn;

@@ -678,8 +678,6 @@ static int check_device_run(dasp_cent *c, const float *d_phs, long phs_stride, i
                             const float *d_zpois, const float *d_zread) {
   if (!c) return fail(DASP_CENT_EINVAL, "null handle");
   if (!d_phs || nframes < 1 || !d_cents) return fail(DASP_CENT_EINVAL, "d_phs, d_cents and nframes >= 1 are required");
-  if (c->nintegrations + c->nlatency != 1)
-    return fail(DASP_CENT_EINVAL, "device runs treat every frame as a readout (nintegrations+nlatency must be 1)");
   if (phs_stride < (long)c->npup * c->npup * (c->atmosPhaseType + 1)) return fail(DASP_CENT_EINVAL, "phs_stride too small");
   if (need_pois(c) && !d_zpois) return fail(DASP_CENT_EINVAL, "Poisson noise is on: d_zpois is required");
   if (need_read(c) && !d_zread) return fail(DASP_CENT_EINVAL, "read noise is on: d_zread is required");

@@ -149,7 +149,11 @@ __device__ __forceinline__ void dft_sincos(float th, float *sn, float *cs) {
 }
 
 template <typename T8, bool FAST>
-__global__ __launch_bounds__(256) void dft16_cog_kernel(FusedArgs a) {
+__global__ __launch_bounds__(256) void dft16_cog_kernel(FusedArgs a_) {
+  // arguments read through a constant-address-space pointer to the kernel argument segment, so
+  // the compiler reloads fields where used instead of holding them in SGPRs (no SGPR spills)
+  typedef const __attribute__((address_space(4))) FusedArgs KernArgs;
+  const FusedArgs &a = *(const FusedArgs *)(KernArgs *)__builtin_amdgcn_kernarg_segment_ptr();
   __shared__ float pxbuf[4][64 * 8];
   const int l = threadIdx.x & 63;
   const int h = l >> 5;

@@ -23,12 +23,17 @@ constexpr int kXcds = 8;
 
 template <int NP, int F, int B, bool FAST, bool SPOT>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NP <= 16 ? 2 : 1)))
-void fused_cog_kernel(FusedArgs a) {
+void fused_cog_kernel(FusedArgs a_) {
+  // Read the arguments through a constant-address-space pointer to the kernel argument segment:
+  // the compiler then reloads fields where they are used instead of holding them in SGPRs
+  // (half the SGPR spills, 4-5 % faster at n16/nfft32, measured).
+  typedef const __attribute__((address_space(4))) FusedArgs KernArgs;
+  const FusedArgs &a = *(const FusedArgs *)(KernArgs *)__builtin_amdgcn_kernarg_segment_ptr();
   using S = FusedShape<NP, F, B>;
   constexpr int SPW = S::SPW, SUB = stage_per_subap<NP, F, SPOT>();
   __shared__ cf stage[S::WAVES][SPW * SUB];
   const int lane = threadIdx.x & 63;
-  const int w = threadIdx.x >> 6;
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int sl = lane / NP;
   const int j = lane - sl * NP;   // pass 1: phase column; pass 2: row group m
   const int xcd = blockIdx.x % kXcds;
@@ -39,19 +44,21 @@ void fused_cog_kernel(FusedArgs a) {
   const long stride = (long)(gridDim.x / kXcds) * S::WAVES;
   long q = (long)(blockIdx.x / kXcds) * S::WAVES + w;
   if (q >= ngx) return;           // wave-uniform
-  auto item_of = [&](long qq) -> long {
+  auto item_of = [&](long qq) -> ItemRef {   // qq is wave-uniform: scalar division
     const long f = qq / nsgx;
-    const long idx = (sg0 + qq - f * nsgx) * SPW + sl;
-    return qq < ngx && idx < a.nactive ? f * a.nactive + idx : -1;
+    return item_at(a, qq < ngx ? (int)f : -1, (int)((sg0 + qq - f * nsgx) * SPW) + sl);
   };
   cf *st = &stage[w][sl * SUB];
   PhaseCol<NP> cur;
-  load_item<NP>(a, item_of(q), j, cur);
+  ItemRef it = item_of(q);
+  load_item<NP>(a, it, j, cur);
   for (; q < ngx; q += stride) {
     PhaseCol<NP> nxt;
-    load_item<NP>(a, item_of(q + stride), j, nxt);   // past the end: zeros
-    fused_item<NP, F, B, FAST, SPOT>(a, item_of(q), j, w, sl, cur, st);
+    const ItemRef itn = item_of(q + stride);
+    load_item<NP>(a, itn, j, nxt);   // past the end: zeros
+    fused_item<NP, F, B, FAST, SPOT>(a, it, j, w, sl, cur, st);
     cur = nxt;
+    it = itn;
   }
 }
 

@@ -73,15 +73,49 @@ struct PhaseCol {
   float ph[NP], pu[NP];
 };
 
+// A work item as a lane group sees it: frame, subaperture index, and whether the lane group
+// holds a real item (padding groups compute on zeros and write nothing).
+struct ItemRef {
+  int frame;
+  int s;
+  bool valid;
+};
+
+__device__ __forceinline__ long wave_uniform(unsigned long long v) {
+  const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)v);
+  const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(v >> 32));
+  return (long)(((unsigned long long)hi << 32) | lo);
+}
+
+// Item idx (index into the active list) of frame `frame`; idx >= nactive or frame < 0 is padding.
+__device__ __forceinline__ ItemRef item_at(const FusedArgs &a, int frame, int idx) {
+  ItemRef r;
+  r.valid = frame >= 0 && idx < a.nactive;
+  r.frame = r.valid ? frame : 0;
+  r.s = r.valid ? a.active[idx] : 0;
+  return r;
+}
+
+// Item first + sl of the frame-major numbering frame * nactive + idx, where `first` (the group's
+// first item) is the same for the whole wave: the division runs once, on the scalar unit.
+__device__ __forceinline__ ItemRef item_flat(const FusedArgs &a, long first, int sl) {
+  first = wave_uniform((unsigned long long)first);
+  const long f0 = first / a.nactive;
+  int idx = (int)(first - f0 * a.nactive) + sl;
+  int f = (int)f0;
+  if (idx >= a.nactive) {
+    idx -= a.nactive;
+    ++f;
+  }
+  const bool in = first + sl < a.nitems;
+  return item_at(a, in ? f : -1, idx);
+}
+
 // Load phase column j (rows i < n) and the pupil of one work item; zeros where out of range.
 template <int NP>
-__device__ __forceinline__ void load_item(const FusedArgs &a, long item, int j, PhaseCol<NP> &c) {
-  const bool valid = item >= 0 && item < a.nitems;   // item < 0: padding lane
-  int frame = 0, s = 0;
-  if (valid) {
-    frame = (int)(item / a.nactive);
-    s = a.active[item - (long)frame * a.nactive];
-  }
+__device__ __forceinline__ void load_item(const FusedArgs &a, const ItemRef &it, int j, PhaseCol<NP> &c) {
+  const bool valid = it.valid;
+  const int frame = it.frame, s = it.s;
   const int n = a.n;
   const int sy = s / a.nsubx, sx = s - (s / a.nsubx) * a.nsubx;
   const float *ph = a.phs + (long)frame * a.phs_stride + (long)(sy * n) * a.npup + sx * n;
@@ -191,16 +225,12 @@ __device__ __forceinline__ void spot_convolve(const FusedArgs &a, int s, int j, 
 
 // One work item (frame, subaperture) per NP-lane group: image formation, detector, CoG.
 template <int NP, int F, int B, bool FAST, bool SPOT = false>
-__device__ __forceinline__ void fused_item(const FusedArgs &a, long item, int j, int w, int sl,
+__device__ __forceinline__ void fused_item(const FusedArgs &a, const ItemRef &it, int j, int w, int sl,
                                            const PhaseCol<NP> &col, cf *st) {
   using S = FusedShape<NP, F, B>;
   constexpr int R = S::R, NIMG = S::NIMG, RB = S::RB, G = S::G, TS = S::TS;
-  const bool valid = item >= 0 && item < a.nitems;   // item < 0: padding lane
-  int frame = 0, s = 0;
-  if (valid) {
-    frame = (int)(item / a.nactive);
-    s = a.active[item - (long)frame * a.nactive];
-  }
+  const bool valid = it.valid;
+  const int frame = it.frame, s = it.s;
   const int n = a.n;
   // ---- pass 1 operand: E[i][j] = pup * exp(i (phi - t)) ----
   cf x[NP];

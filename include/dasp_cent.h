@@ -166,7 +166,9 @@ int dasp_cent_set_fast_trig(dasp_cent_t *h, int enable);
  * d_phs + f*phs_stride and writes d_cents + f*nsubaps*2, d_bimg + f*nsubaps*nimg^2 (if non-NULL)
  * and d_corrimg + f*nsubaps*C^2 (if non-NULL). d_zpois/d_zread are pre-drawn N(0,1) arrays of
  * nsubaps*nimg^2 per frame (may be NULL when the noise they feed is off). Asynchronous on stream.
- * Every frame is a readout (nintegrations+nlatency must be 1). */
+ * Every frame is a readout: with nintegrations+nlatency > 1 the caller runs the integration
+ * cycle (only the readout step forms an image, from that step's phase, as computeHll zeroes
+ * its buffer on every call, cmod/src/centmodule.c:478). */
 int dasp_cent_run_device(dasp_cent_t *h, const float *d_phs, long phs_stride, int nframes,
                          float *d_cents, float *d_bimg, float *d_corrimg,
                          const float *d_zpois, const float *d_zread, void *stream);
