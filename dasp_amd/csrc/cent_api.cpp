@@ -95,7 +95,7 @@ struct dasp_cent {
   int nsubx = 0, nsubaps = 0, n = 0, F = 0, clip = 0, nimg = 0, ncen = 0;
   int psf = 0, padded = 0, pbf = 1, preBinOrig = 1, spotDim = 0;
   int corrsize = 0, corrPatternSize = 0, npup = 0;
-  int atmosPhaseType = 0, interpolate = 0;
+  int atmosPhaseType = 0, interpolate = 0, inputPhaseSize = 0;
   int curInteg = 0, form = DASP_CENT_FORM_FFT32;
   int fastTrig = 1;   // reduced-argument hardware sin/cos (max abs error 3.8e-7, measured)
   unsigned long gen = 0;   // bumped by every update/set_* call: step graphs built on it are stale
@@ -127,6 +127,7 @@ struct dasp_cent {
   DevBuf d_calBounds, d_calSteps, d_calCoeff, d_ubArr, d_input, d_subapLoc, d_cents, d_bimg;
   DevBuf d_corrimg, d_hll, d_zpois, d_zread, d_bscratch, d_cscratch;
   DevBuf d_otfk;   // spot kernel table of the fused kernels (psfsize == fftsize)
+  DevBuf d_interp; // resampled phase grids when the input grid is not nsubx*phasesize
   std::vector<float> h_stage;
   std::vector<float> noiseHost;
   // kernel timing
@@ -520,6 +521,21 @@ static int launch_phase(dasp_cent *c, const float *d_phs, long phs_stride, int n
                         hipStream_t st) {
   int rc = upload_tables(c);
   if (rc) return rc;
+  if (c->interpolate) {   // interpolatePhase (:1628-1758) onto the nsubx*phasesize grid
+    const long tot2 = (long)c->npup * c->npup;
+    HIPCHK(c->d_interp.ensure((size_t)nframes * tot2 * 4));
+    InterpArgs ia{};
+    ia.in = d_phs;
+    ia.in_stride = phs_stride;
+    ia.m = c->inputPhaseSize;
+    ia.out = c->d_interp.as<float>();
+    ia.tot = c->npup;
+    ia.scale = (float)((c->inputPhaseSize - 1.) / (c->npup - 1));
+    ia.nframes = nframes;
+    HIPCHK(launch_interp(ia, st));
+    d_phs = c->d_interp.as<float>();
+    phs_stride = tot2;
+  }
   const long nitems = (long)nframes * (long)c->active.size();
   const int np = pad_pow2(c->n);
   const float *amp = c->atmosPhaseType == 1 ? d_phs + (long)c->npup * c->npup : nullptr;
@@ -678,7 +694,8 @@ static int check_device_run(dasp_cent *c, const float *d_phs, long phs_stride, i
                             const float *d_zpois, const float *d_zread) {
   if (!c) return fail(DASP_CENT_EINVAL, "null handle");
   if (!d_phs || nframes < 1 || !d_cents) return fail(DASP_CENT_EINVAL, "d_phs, d_cents and nframes >= 1 are required");
-  if (phs_stride < (long)c->npup * c->npup * (c->atmosPhaseType + 1)) return fail(DASP_CENT_EINVAL, "phs_stride too small");
+  const long side = c->interpolate ? c->inputPhaseSize : c->npup;
+  if (nframes > 1 && phs_stride < side * side * (c->atmosPhaseType + 1)) return fail(DASP_CENT_EINVAL, "phs_stride too small");
   if (need_pois(c) && !d_zpois) return fail(DASP_CENT_EINVAL, "Poisson noise is on: d_zpois is required");
   if (need_read(c) && !d_zread) return fail(DASP_CENT_EINVAL, "read noise is on: d_zread is required");
   return 0;
@@ -694,6 +711,7 @@ static int device_scratch(dasp_cent *c, int nframes, float **bimg, float **corr)
     HIPCHK(c->d_cscratch.ensure((size_t)nframes * c->nsubaps * c->corrsize * c->corrsize * 4));
     *corr = c->d_cscratch.as<float>();
   }
+  if (c->interpolate) HIPCHK(c->d_interp.ensure((size_t)nframes * c->npup * c->npup * 4));
   if (!use_fused(c) && c->active.size() > 0) {   // image-formation scratch of the general path
     const long hll = (long)nframes * (long)c->active.size() * c->padded * c->padded;
     HIPCHK(c->d_hll.ensure(hll * sizeof(float)));
@@ -735,7 +753,7 @@ struct dasp_cent_batch {
 };
 
 static bool batch_fusable(dasp_cent *c) {
-  return use_fused(c) && !use_dft(c) && !corr_stage(c) &&
+  return use_fused(c) && !use_dft(c) && !corr_stage(c) && !c->interpolate &&
          batch_supported(pad_pow2(c->n), c->F, c->nimg, c->spotDim != 0);
 }
 
@@ -994,8 +1012,15 @@ int dasp_cent_initialise(const dasp_cent_params *pp, dasp_cent_t **out) {
     else if (p.phs.ndim == 3) c->atmosPhaseType = 1;
     else return bail(fail(DASP_CENT_EINVAL, "centmodule - phase should be 2D or 3D (for phase+amp)"));
     const int o = c->atmosPhaseType;
-    if (p.phs.dims[o] != c->npup || p.phs.dims[o + 1] != c->npup)
-      return bail(fail(DASP_CENT_EUNSUPPORTED, "phase grid %ldx%ld differs from nsubx*phasesize=%d: bicubic interpolation (interpolatePhase) is not implemented on the device", p.phs.dims[o], p.phs.dims[o + 1], c->npup));
+    if (p.phs.dims[o] != p.phs.dims[o + 1]) return bail(fail(DASP_CENT_EINVAL, "centmodule - phase should be square"));
+    c->inputPhaseSize = (int)p.phs.dims[o];
+    if (c->inputPhaseSize != c->npup) {
+      // interpolatePhase (:1628-1758): resampled onto the nsubx*phasesize grid on the device
+      if (o == 1)
+        return bail(fail(DASP_CENT_EUNSUPPORTED, "phase+amplitude on a grid other than nsubx*phasesize (%d) is not supported", c->npup));
+      if (c->inputPhaseSize < 3) return bail(fail(DASP_CENT_EINVAL, "phase grid too small to interpolate"));
+      c->interpolate = 1;
+    }
     if (o == 1 && p.phs.dims[0] != 2) return bail(fail(DASP_CENT_EINVAL, "phase+amplitude should be (2,npup,npup)"));
     c->h_phs = (float *)p.phs.data;
     c->phsElems = numel(p.phs);
@@ -1100,7 +1125,7 @@ int dasp_cent_free(dasp_cent_t *c) {
                     &c->d_corrPat, &c->d_centWeight, &c->d_refCents, &c->d_calData, &c->d_calBounds,
                     &c->d_calSteps, &c->d_calCoeff, &c->d_ubArr, &c->d_input, &c->d_subapLoc, &c->d_cents,
                     &c->d_bimg, &c->d_corrimg, &c->d_hll, &c->d_zpois, &c->d_zread, &c->d_bscratch, &c->d_cscratch,
-                    &c->d_otfk};
+                    &c->d_otfk, &c->d_interp};
   for (DevBuf *b : bufs) b->release();
   delete c;
   return 0;

@@ -168,6 +168,18 @@ struct CorrArgs {
   float *corrimg;
   long corr_stride;
 };
+// interpolatePhase on the whole output grid (cent_interp.hip).
+struct InterpArgs {
+  const float *in;       // (m, m) per frame
+  long in_stride;
+  int m;
+  float *out;            // (tot, tot) per frame, contiguous frames
+  int tot;               // nsubx * phasesize
+  float scale;           // (m - 1) / (tot - 1), float as the reference stores it
+  int nframes;
+};
+hipError_t launch_interp(const InterpArgs &a, hipStream_t st);
+
 bool corr_fit_supported(int n);
 hipError_t launch_corr_fit(const CorrArgs &a, int n, bool corr, hipStream_t st);
 

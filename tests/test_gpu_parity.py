@@ -279,3 +279,20 @@ def test_dft_mfma_form_error_vs_fp64(form):
     if form == "fp16":
         assert d.max() < 1e-4
     c.free()
+
+
+@pytest.mark.parametrize("m", [45, 83])
+def test_interpolated_phase_grid(m):
+    """Phase on an m x m grid (not nsubx*n = 64): interpolatePhase (bicubic) onto the 64 x 64
+    grid before imaging, against the oracle's FP64 restatement."""
+    say("case interpolatePhase m=%d" % m)
+    from dasp_amd import synth
+    phs = synth.phase_screen(m, 4.2, 0.15, 30.0, 77, wavelength_scale=500. / 640.) * 0.7
+    c = Case(8, 8, 16, 8, obs=0.25, phs=phs.astype(np.float32))
+    oc, ob, _ = c.run_both()
+    act = _active(c)
+    d = report("interp m=%d" % m, c.cents, oc, act)
+    assert d.max() < 1e-4
+    b = c.bimg.reshape(-1, 8, 8)[act]
+    np.testing.assert_allclose(b, ob[act], rtol=0, atol=2e-5 * ob[act].max())
+    c.free()

@@ -362,3 +362,20 @@ def test_gaussian_fit_vertex_and_in_place_log():
     act = flag.ravel() == 1
     want = np.where(b0 < 1e-3, -12.0, np.log(np.maximum(b0, 1e-300)))
     np.testing.assert_allclose(b1[act], want[act], rtol=1e-12, atol=1e-12)
+
+
+def test_interpolate_phase_edges_and_samples():
+    """Edges of interpolatePhase: the output corners are the input corners (the last sample is
+    taken at t = 1, :1653-1656), a constant grid stays constant through the replicated edge
+    rows/columns, and where the output grid hits input samples exactly (m - 1 a multiple of
+    tot - 1) the interpolation returns the samples."""
+    rng = np.random.default_rng(9)
+    for m, nsubx, n in ((13, 2, 4), (9, 3, 3), (40, 4, 8)):
+        inp = rng.normal(size=(m, m))
+        out = O.interpolate_phase(inp, nsubx, n)
+        for (a, b) in ((0, 0), (0, -1), (-1, 0), (-1, -1)):
+            assert abs(out[a, b] - inp[a, b]) < 1e-12
+        np.testing.assert_allclose(O.interpolate_phase(np.full((m, m), 2.5), nsubx, n), 2.5, rtol=0, atol=1e-12)
+    inp = rng.normal(size=(25, 25))
+    out = O.interpolate_phase(inp, 1, 13)   # tot = 13, scale = 2: output k samples input 2k
+    np.testing.assert_allclose(out, inp[::2, ::2], rtol=0, atol=1e-12)
