@@ -12,6 +12,7 @@
 // The fit's window sums run in one lane in the order of the general kernel.
 #include "cent_internal.h"
 #include "fft_reg.h"
+#include "wave_ops.h"
 
 namespace dasp {
 
@@ -23,39 +24,22 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-template <int W>
-__device__ __forceinline__ float gsum(float v) {
-  static_for<0, 6>([&](auto b) {
-    constexpr int m = 1 << decltype(b)::value;
-    if constexpr (m < W) v += __shfl_xor(v, m);
-  });
-  return v;
-}
-
-template <int W>
-__device__ __forceinline__ float gmax(float v) {
-  static_for<0, 6>([&](auto b) {
-    constexpr int m = 1 << decltype(b)::value;
-    if constexpr (m < W) v = fmaxf(v, __shfl_xor(v, m));
-  });
-  return v;
-}
-
 // (value, index) maximum with the first index winning ties; value 0 / index 0 when nothing is
 // positive (the reference's scan starts from mx = 0 with a strict comparison).
 template <int W>
 __device__ __forceinline__ void gargmax(float &v, int &i) {
-  static_for<0, 6>([&](auto b) {
-    constexpr int m = 1 << decltype(b)::value;
-    if constexpr (m < W) {
-      const float ov = __shfl_xor(v, m);
-      const int oi = __shfl_xor(i, m);
-      if (ov > v || (ov == v && oi < i)) {
-        v = ov;
-        i = oi;
-      }
+  auto step = [&](float ov, int oi) {
+    if (ov > v || (ov == v && oi < i)) {
+      v = ov;
+      i = oi;
     }
-  });
+  };
+  auto movi = [](int x, auto ctrl) { return __builtin_amdgcn_mov_dpp(x, decltype(ctrl)::value, 0xF, 0xF, false); };
+  if constexpr (W >= 2) step(dpp_mov<dpp::kQuadXor1>(v), movi(i, std::integral_constant<int, dpp::kQuadXor1>{}));
+  if constexpr (W >= 4) step(dpp_mov<dpp::kQuadXor2>(v), movi(i, std::integral_constant<int, dpp::kQuadXor2>{}));
+  if constexpr (W >= 8) step(dpp_mov<dpp::kRowHalfMirror>(v), movi(i, std::integral_constant<int, dpp::kRowHalfMirror>{}));
+  if constexpr (W >= 16) step(dpp_mov<dpp::kRowMirror>(v), movi(i, std::integral_constant<int, dpp::kRowMirror>{}));
+  if constexpr (W >= 32) step(__shfl_xor(v, 16), __shfl_xor(i, 16));
 }
 
 }  // namespace
@@ -116,7 +100,7 @@ __global__ __launch_bounds__(256) void corr_fit_kernel(CorrArgs a) {
       if (t <= 1.f && t > 0.f) {
         float mx = 0.f;
         static_for<0, N>([&](auto c) { mx = fmaxf(mx, row[decltype(c)::value]); });
-        t = gmax<N>(mx) * t;
+        t = group_max<N>(mx) * t;
       }
       static_for<0, N>([&](auto c) {
         const float v = row[decltype(c)::value];
@@ -137,7 +121,7 @@ __global__ __launch_bounds__(256) void corr_fit_kernel(CorrArgs a) {
         mi = r * N + c;
       }
     });
-    sum = gsum<N>(sum);
+    sum = group_sum<N>(sum);
     gargmax<N>(mv, mi);
     const int mxpos = mv > 0.f ? mi : 0;
     float cx = 0.f, cy = 0.f;
@@ -242,9 +226,9 @@ __global__ __launch_bounds__(256) void corr_fit_kernel(CorrArgs a) {
           }
         });
       }
-      sw = gsum<N>(sw);
-      sxw = gsum<N>(sxw);
-      syw = gsum<N>(syw);
+      sw = group_sum<N>(sw);
+      sxw = group_sum<N>(sxw);
+      syw = group_sum<N>(syw);
       if (sw > 0.f) {
         cx = sxw / sw + sx - (N - 1) / 2.f;
         cy = syw / sw + sy - (N - 1) / 2.f;

@@ -19,6 +19,7 @@
 // follow as in the fused FP32 kernel.
 #include "cent_internal.h"
 #include "noise.h"
+#include "wave_ops.h"
 
 namespace dasp {
 
@@ -32,13 +33,24 @@ struct Split;
 
 template <>
 struct Split<h8> {
+  // hi: round-toward-zero pair conversion (one v_cvt_pkrtz per two values) stored straight into
+  // the operand's dwords; lo = x - hi is exact in FP32 and is rounded to FP16 by a mixed-precision
+  // FMA, fma(hi, -1, x) (v_fma_mix, one instruction per value)
   static __device__ __forceinline__ void run(const float (&x)[8], h8 &hi, h8 &lo) {
+    typedef unsigned u4 __attribute__((ext_vector_type(4)));
+    typedef _Float16 h2 __attribute__((ext_vector_type(2)));
+    u4 hw, lw;
 #pragma unroll
-    for (int t = 0; t < 8; ++t) {
-      const _Float16 hv = (_Float16)x[t];
-      hi[t] = hv;
-      lo[t] = (_Float16)(x[t] - (float)hv);
+    for (int t = 0; t < 8; t += 2) {
+      const h2 hp = __builtin_bit_cast(h2, __builtin_amdgcn_cvt_pkrtz(x[t], x[t + 1]));
+      h2 lp;
+      lp[0] = (_Float16)__builtin_fmaf((float)hp[0], -1.0f, x[t]);
+      lp[1] = (_Float16)__builtin_fmaf((float)hp[1], -1.0f, x[t + 1]);
+      hw[t / 2] = __builtin_bit_cast(unsigned, hp);
+      lw[t / 2] = __builtin_bit_cast(unsigned, lp);
     }
+    hi = __builtin_bit_cast(h8, hw);
+    lo = __builtin_bit_cast(h8, lw);
   }
 };
 
@@ -121,17 +133,6 @@ __device__ __forceinline__ void cgemm_cv(const CConst<T8> &A, const COp<T8> &B, 
   ci = mfma(A.il, B.rh, ci);
 }
 
-__device__ __forceinline__ float wave_sum(float v) {
-#pragma unroll
-  for (int m = 1; m < 64; m <<= 1) v += __shfl_xor(v, m);
-  return v;
-}
-__device__ __forceinline__ float wave_max(float v) {
-#pragma unroll
-  for (int m = 1; m < 64; m <<= 1) v = fmaxf(v, __shfl_xor(v, m));
-  return v;
-}
-
 constexpr int kN = 16, kF = 32, kM = 16;   // phase size, fft size, detector size (bin 2)
 
 template <bool FAST>
@@ -175,35 +176,61 @@ __global__ __launch_bounds__(256) void dft16_cog_kernel(FusedArgs a_) {
     A2.set(re2, im2);
   }
   const long npairs = (a.nitems + 1) / 2;
-  const long wave_id = (long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  const long wave_id = (long)blockIdx.x * (blockDim.x >> 6) + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const long nwaves = (long)gridDim.x * (blockDim.x >> 6);
   const int n = a.n;
   const int jj = c & 15;
-  // stage-1 operand of a pair: lane holds E_s[i][j], s = c/16, j = c%16, i = 8h + t
-  auto load_pair = [&](long pr, float (&ph8)[8], float (&pu8)[8]) {
-    const long it = 2 * pr + (c >> 4);
-    const bool v = pr < npairs && it < a.nitems;
-    int fr = 0, sa = 0;
-    if (v) {
-      fr = (int)(it / a.nactive);
-      sa = a.active[it - (long)fr * a.nactive];
+  // frame and active index of a pair's first item (the pair index is wave-uniform: one scalar
+  // division per pair)
+  auto pair_ref = [&](long pr, int &f0, int &idx0) {
+    const long first = 2 * pr;
+    const long q = first / a.nactive;
+    f0 = (int)q;
+    idx0 = (int)(first - q * a.nactive);
+  };
+  auto next_item = [&](int f0, int idx0, int u, int &f, int &idx) {
+    idx = idx0 + u;
+    f = f0;
+    if (idx >= a.nactive) {
+      idx -= a.nactive;
+      ++f;
     }
+  };
+  // stage-1 operand of a pair: lane holds E_s[i][j], s = c/16, j = c%16, i = 8h + t
+  auto load_pair = [&](long pr, int f0, int idx0, float (&ph8)[8], float (&pu8)[8]) {
+    const int u = c >> 4;
+    int fr, idx;
+    next_item(f0, idx0, u, fr, idx);
+    const bool v = pr < npairs && 2 * pr + u < a.nitems;
+    const int sa = v ? a.active[idx] : 0;
+    fr = v ? fr : 0;
     const int sy = sa / a.nsubx, sx = sa - (sa / a.nsubx) * a.nsubx;
-    const float *ph = a.phs + (long)fr * a.phs_stride + (long)(sy * n) * a.npup + sx * n;
-    const float *pu = a.pup + (long)sa * n * n;
+    const float *ph = a.phs + (long)fr * a.phs_stride + (long)(sy * n) * a.npup + sx * n + jj;
+    const float *pu = a.pup + (long)sa * n * n + jj;
 #pragma unroll
     for (int t = 0; t < 8; ++t) {
       const int i = 8 * h + t;
       const bool in = v && i < n && jj < n;
-      ph8[t] = in ? ph[(long)i * a.npup + jj] : 0.f;
-      pu8[t] = in ? pu[i * n + jj] : 0.f;
+      ph8[t] = in ? ph[(long)i * a.npup] : 0.f;
+      pu8[t] = in ? pu[i * n] : 0.f;
     }
   };
   float cph[8], cpu[8];
-  load_pair(wave_id, cph, cpu);
+  int cf0, cidx0;
+  pair_ref(wave_id, cf0, cidx0);
+  // the pair index advances by nwaves: frame / active index advance by a fixed step (no division
+  // inside the loop)
+  int stepF, stepR;
+  pair_ref(nwaves, stepF, stepR);
+  load_pair(wave_id, cf0, cidx0, cph, cpu);
   for (long pr = wave_id; pr < npairs; pr += nwaves) {
     float nph[8], npu[8];
-    load_pair(pr + nwaves, nph, npu);   // next pair's loads overlap this pair's work
+    int nf0 = cf0 + stepF, nidx0 = cidx0 + stepR;
+    if (nidx0 >= a.nactive) {
+      nidx0 -= a.nactive;
+      ++nf0;
+    }
+    load_pair(pr + nwaves, nf0, nidx0, nph, npu);   // next pair's loads overlap this pair's work
     float er[8], ei[8];
 #pragma unroll
     for (int t = 0; t < 8; ++t) {
@@ -219,6 +246,9 @@ __global__ __launch_bounds__(256) void dft16_cog_kernel(FusedArgs a_) {
       cph[t] = nph[t];
       cpu[t] = npu[t];
     }
+    const int pf0 = cf0, pidx0 = cidx0;
+    cf0 = nf0;
+    cidx0 = nidx0;
     COp<T8> A1;
     A1.set(er, ei);
     f16v zr = {}, zi = {};
@@ -226,10 +256,10 @@ __global__ __launch_bounds__(256) void dft16_cog_kernel(FusedArgs a_) {
 
 #pragma unroll
     for (int sub = 0; sub < 2; ++sub) {
-      const long item = 2 * pr + sub;
-      if (item >= a.nitems) break;   // wave-uniform
-      const int frame = (int)(item / a.nactive);
-      const int s = a.active[item - (long)frame * a.nactive];
+      if (2 * pr + sub >= a.nitems) break;   // wave-uniform
+      int frame, sidx;
+      next_item(pf0, pidx0, sub, frame, sidx);
+      const int s = a.active[sidx];
       // ---- stage 2: B2 = Z_s^T from accumulator registers 8*sub .. 8*sub+7 ----
       float br[8], bi[8];
 #pragma unroll
@@ -247,13 +277,14 @@ __global__ __launch_bounds__(256) void dft16_cog_kernel(FusedArgs a_) {
 #pragma unroll
       for (int q = 0; q < 8; ++q) {
         const int r = 2 * q;
-        float v = fmaf(xr[r], xr[r], xi[r] * xi[r]) + fmaf(xr[r + 1], xr[r + 1], xi[r + 1] * xi[r + 1]);
-        v += __shfl_xor(v, 1);
+        float v = fmaf(xr[r], xr[r], fmaf(xi[r], xi[r], fmaf(xr[r + 1], xr[r + 1], xi[r + 1] * xi[r + 1])));
+        v += dpp_mov<dpp::kQuadXor1>(v);   // the lane pair (k, k^1) of one detector row
         pix[q] = v;
       }
-      const bool owner = (c & 1) == 0;
-      // detector row of the lane pair (k = c, c+1): hll rows F-1-k, F-2-k -> I = (F-2-c)/2
-      const int I = (kF - 2 - c) >> 1;
+      // both lanes of a pair hold the pair's sums: lane parity p takes pixel columns q = 4p..4p+3
+      const int par = c & 1;
+      // detector row of the lane pair (k = c, c+1): hll rows F-1-k, F-2-k -> I = (F-2-(c & ~1))/2
+      const int I = (kF - 2 - (c & ~1)) >> 1;
       const int M2 = kM * kM;
       const long pix0 = (long)s * M2;
       const float scale = a.scale[s];
@@ -264,19 +295,21 @@ __global__ __launch_bounds__(256) void dft16_cog_kernel(FusedArgs a_) {
         const float add = (skyadd > 0.f ? skyadd : 0.f) + (a.threshType >= 0 && a.threshType <= 2 ? a.readbg : 0.f);
         const float thr = a.floor;
 #pragma unroll
-        for (int q = 0; q < 8; ++q) {
+        for (int qq = 0; qq < 4; ++qq) {
           // l pair starts at l0 = 8(q/2) + 4h + 2(q%2): detector column J = (F-2-l0)/2
+          const int q = 4 * par + qq;
           const int J = (kF - 2 - (8 * (q >> 1) + 4 * h + 2 * (q & 1))) >> 1;
-          float v = pix[q] * scale + add;
+          float v = (par ? pix[4 + qq] : pix[qq]) * scale + add;
           if (a.threshType == 1) v = v < thr ? 0.f : v;
           else if (a.threshType == 0) v = v < thr ? 0.f : v - thr;
-          if (owner && I >= st0 && I < en0 && J >= st0 && J < en0) {
+          if (I >= st0 && I < en0 && J >= st0 && J < en0) {
             sw += v;
             sxw += v * J;
             syw += v * I;
           }
         }
       } else {
+        const bool owner = par == 0;
 #pragma unroll
         for (int q = 0; q < 8; ++q) px[q] = pix[q];
         const long noff = (long)frame * a.noise_stride + pix0;
@@ -299,7 +332,7 @@ __global__ __launch_bounds__(256) void dft16_cog_kernel(FusedArgs a_) {
         }
         float thr = a.floor;
         if (a.threshType == 2) {
-          mx = wave_max(mx);
+          mx = wave_maximum(mx);
           if (a.floor > 0.f && a.floor <= 1.f) thr = a.floor * fmaxf(mx, 0.f);
         }
         float *bout = a.bimg ? a.bimg + (long)frame * a.bimg_stride + pix0 : nullptr;
@@ -323,14 +356,17 @@ __global__ __launch_bounds__(256) void dft16_cog_kernel(FusedArgs a_) {
           }
         }
       }
-      sw = wave_sum(sw);
-      sxw = wave_sum(sxw);
-      syw = wave_sum(syw);
+      sw = wave_total(sw);
+      sxw = wave_total(sxw);
+      syw = wave_total(syw);
       if (l == 0 && !a.no_cents) {
         float cx = 0.f, cy = 0.f;
         if (sw > 0.f) {
-          cx = sxw / sw - (kM - 1) / 2.f;
-          cy = syw / sw - (kM - 1) / 2.f;
+          // hardware reciprocal (1 ulp) instead of two IEEE divisions: a few ulp on the slopes,
+          // far inside this form's hi/lo error
+          const float rw = __builtin_amdgcn_rcpf(sw);
+          cx = sxw * rw - (kM - 1) / 2.f;
+          cy = syw * rw - (kM - 1) / 2.f;
         }
         if (a.refCents) {
           cx -= a.refCents[s * 2];
@@ -350,7 +386,9 @@ hipError_t launch_dft(const FusedArgs &a, int bf16, hipStream_t st) {
   if (a.nitems <= 0) return hipSuccess;
   const long npairs = (a.nitems + 1) / 2;
   long blocks = (npairs + 3) / 4;
-  const long cap = 256L * 8 * 4;   // enough resident waves to fill the chip, then grid-stride
+  // persistent grid: every wave builds the DFT-matrix operands once (hundreds of instructions),
+  // so launch about the resident wave count (two waves per SIMD) and grid-stride
+  const long cap = 256L * 2;
   if (blocks > cap) blocks = cap;
   if (bf16) {
     if (a.fast_trig) hipLaunchKernelGGL((dft16_cog_kernel<b8, true>), dim3((unsigned)blocks), dim3(256), 0, st, a);

@@ -12,6 +12,7 @@
 #include "cent_internal.h"
 #include "fft_reg.h"
 #include "noise.h"
+#include "wave_ops.h"
 
 namespace dasp {
 
@@ -32,24 +33,6 @@ __device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
-template <int W>
-__device__ __forceinline__ float group_sum(float v) {
-  static_for<0, 6>([&](auto b) {
-    constexpr int m = 1 << decltype(b)::value;
-    if constexpr (m < W) v += __shfl_xor(v, m);
-  });
-  return v;
-}
-
-template <int W>
-__device__ __forceinline__ float group_max(float v) {
-  static_for<0, 6>([&](auto b) {
-    constexpr int m = 1 << decltype(b)::value;
-    if constexpr (m < W) v = fmaxf(v, __shfl_xor(v, m));
-  });
-  return v;
 }
 
 // exp(i th) for the phase argument. Accurate path: sincosf. Fast path: Cody-Waite reduction by
