@@ -47,6 +47,15 @@ def measured_traffic(workload, form):
     return None
 
 
+def dft_mfma_per_image(w):
+    return 18 if w.nfft == 32 else 144
+
+
+def dft_form_shape(w):
+    """Geometries the DFT-GEMM forms cover."""
+    return w.nimg == 16 and ((w.n == 16 and w.nfft == 32) or (16 < w.n <= 32 and w.nfft == 64))
+
+
 def roofline(form, w, nact, frames, kavg, abytes, noise):
     """Roofline entry of the main kernel: HBM for the FP32 form, MFMA for the DFT-GEMM forms."""
     ach = abytes / kavg / 1e9
@@ -55,8 +64,9 @@ def roofline(form, w, nact, frames, kavg, abytes, noise):
     if form == "fft32":
         return {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
                 "traffic": traffic, "kernel_avg_ms": kavg * 1e3, "algorithmic_bytes": abytes}
-    # DFT-GEMM form: 18 v_mfma_f32_32x32x16 per subaperture image (hi/lo split, 3 products)
-    flops = 18 * 2 * 32 * 32 * 16 * nact * frames
+    # DFT-GEMM form, v_mfma_f32_32x32x16 per subaperture image (hi/lo split, 3 products):
+    # 18 for n = 16 / nfft = 32, 144 for n <= 32 / nfft = 64
+    flops = dft_mfma_per_image(w) * 2 * 32 * 32 * 16 * nact * frames
     tf = flops / kavg / 1e12
     return {"bound": "mfma", "achieved": tf, "peak": MFMA_PEAK_TFLOPS[form], "unit": "TFLOP/s",
             "frac": tf / MFMA_PEAK_TFLOPS[form], "traffic": traffic, "kernel_avg_ms": kavg * 1e3,
@@ -113,7 +123,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--accurate-trig", action="store_true", help="sincosf instead of the reduced hardware sin/cos")
     ap.add_argument("--no-compare-forms", dest="compare_forms", action="store_false",
-                    help="skip timing the other compute forms (80x80 nfft=32 only)")
+                    help="skip timing the other compute forms (geometries the DFT-GEMM forms cover)")
     ap.add_argument("--calib-mib", type=int, default=0,
                     help="after timing, copy a buffer of this many MiB once (known-size kernel for scaling "
                          "rocprofv3 FETCH_SIZE/WRITE_SIZE counters)")
@@ -260,8 +270,7 @@ def main():
         d = (c0 - oc.reshape(-1, 2))[act]
         result["rms_err_vs_fp64_px"] = float(np.sqrt((d ** 2).mean()))
         result["max_err_vs_fp64_px"] = float(np.abs(d).max())
-    if args.compare_forms and wb is None and not noise and not w.correlation and w.n == 16 and w.nfft == 32 \
-            and w.nimg == 16:
+    if args.compare_forms and wb is None and not noise and dft_form_shape(w):
         # the headline compares the plain FP32 form with the MFMA DFT-GEMM forms on the same input
         forms = {}
         for form in ("fft32", "fp16", "bf16"):

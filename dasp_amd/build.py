@@ -15,7 +15,7 @@ OBJDIR = os.path.join(CSRC, "_build")
 LIB = os.path.join(HERE, "libdasp_cent.so")
 ARCH = "gfx950"
 
-SOURCES = ["cent_form.hip", "cent_backend.hip", "cent_fused.hip", "cent_dft.hip", "cent_batch.hip", "cent_batch32.hip", "cent_corr.hip", "cent_interp.hip",
+SOURCES = ["cent_form.hip", "cent_backend.hip", "cent_fused.hip", "cent_dft.hip", "cent_dft32.hip", "cent_batch.hip", "cent_batch32.hip", "cent_corr.hip", "cent_interp.hip",
            "cent_api.cpp", "host_math.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
@@ -27,7 +27,8 @@ FLAGS = ["-O3", "-std=c++17", "-fPIC", "--offload-arch=" + ARCH,
 
 # Per-file extras. The DFT-GEMM kernels keep their matrix-core accumulators in ordinary VGPRs
 # (no AGPR copies before the accumulators are reused as operands; same occupancy).
-EXTRA = {"cent_dft.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"]}
+EXTRA = {"cent_dft.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"],
+         "cent_dft32.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"]}
 
 
 def _compile(src):

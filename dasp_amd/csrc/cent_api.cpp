@@ -458,7 +458,8 @@ static bool fused_options(dasp_cent *c) {
          !c->haveCalData && !c->haveCalCoeff && (c->calsource != 0 || (c->useBrightest == 0 && !c->haveUBArr));
 }
 
-// The DFT-GEMM kernel covers the fused options for n = 16, fftsize = 32, nimg = 16.
+// The DFT-GEMM kernels cover the fused options for n = 16 / fftsize 32 and n in 17..32 /
+// fftsize 64, nimg 16.
 static bool use_dft(dasp_cent *c) {
   return (c->form == DASP_CENT_FORM_DFT_FP16 || c->form == DASP_CENT_FORM_DFT_BF16) && fused_options(c) &&
          c->spotDim == 0 && dft_supported(c->n, c->F, c->nimg);
@@ -1137,8 +1138,8 @@ int dasp_cent_set_form(dasp_cent_t *c, int form) {
   if (form < DASP_CENT_FORM_FFT32 || form > DASP_CENT_FORM_FFT32_GENERAL) return fail(DASP_CENT_EINVAL, "unknown form %d", form);
   if ((form == DASP_CENT_FORM_DFT_FP16 || form == DASP_CENT_FORM_DFT_BF16) &&
       !(fused_options(c) && c->spotDim == 0 && dft_supported(c->n, c->F, c->nimg)))
-    return fail(DASP_CENT_EUNSUPPORTED, "the DFT-GEMM form covers n=16, fftsize=clipsize=32, nimg=16 with a (weighted) CoG, "
-                                        "no spot kernel, phase only");
+    return fail(DASP_CENT_EUNSUPPORTED, "the DFT-GEMM form covers n=16 with fftsize=clipsize=32 and n=17..32 with "
+                                        "fftsize=clipsize=64, nimg=16, the fused options, no spot kernel, phase only");
   c->form = form;
   return 0;
 }

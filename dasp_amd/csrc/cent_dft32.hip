@@ -1,0 +1,287 @@
+// cent_dft32.hip - the DFT-GEMM form for phase size n <= 32, fftsize 64, nimg 16 (4x4 binning).
+//
+// One sub-aperture per wave. computeHll's zero-padded 2-D transform (cmod/src/centmodule.c:
+// 473-509) as two GEMMs with W[k][i] = exp(-2 pi i k i / 64), on v_mfma_f32_32x32x16_{f16,bf16}
+// with the three-product hi/lo scheme of dft_ops.h:
+//   stage 1: Z^T = E^T W^T   M = 32 (j), N = 64 (k: two tiles kt), K = 32 (i: two steps ks)
+//   stage 2: X^T = W Z^T     M = 64 (l: two tiles lt), N = 64 (k: two tiles kt), K = 32 (j)
+// 48 + 96 = 144 MFMA per image. Stage 1's accumulator of tile kt holds Z^T[j][k] with lane
+// k - 32 kt + 32 h and register r <-> j = 8(r/4) + 4h + r%4, so registers 8 ks .. 8 ks + 7 are
+// the eight K slots of stage 2's B operand for K step ks when W's K index is permuted the same
+// way (slot q <-> j = 16 ks + 8(q/4) + 4h + q%4): no transpose. Stage 2's accumulator of tile
+// (lt, kt) holds X[k][l] with lane = k - 32 kt + 32 h and register 4g + u <-> l = 32 lt + 8g + 4h
+// + u: a 4x4 detector pixel is four registers in-lane (u) summed over the four lanes of a quad
+// (k), and after the 180-degree map it is pixel (I, J) = (15 - 8 kt - c/4, 15 - 8 lt - 2g - h).
+// The DFT-matrix operands (eight of them) are built once per workgroup into LDS.
+#include "dft_ops.h"
+#include "noise.h"
+#include "wave_ops.h"
+
+namespace dasp {
+
+namespace {
+constexpr int kF64 = 64, kM64 = 16;
+}
+
+template <typename T8, bool FAST>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void dft32_cog_kernel(FusedArgs a_) {
+  typedef const __attribute__((address_space(4))) FusedArgs KernArgs;
+  const FusedArgs &a = *(const FusedArgs *)(KernArgs *)__builtin_amdgcn_kernarg_segment_ptr();
+  constexpr int F = kF64, M = kM64;
+  __shared__ CConst<T8> w1[2][2][64];   // [kt][ks]: W[32 kt + c][16 ks + 8h + t]
+  __shared__ CConst<T8> w2[2][2][64];   // [lt][ks]: W[32 lt + c][16 ks + 8(t/4) + 4h + t%4]
+  __shared__ float pxbuf[4][64 * 4];
+  const int l = threadIdx.x & 63;
+  const int h = l >> 5;
+  const int c = l & 31;
+  float *px = &pxbuf[threadIdx.x >> 6][l * 4];
+  if (threadIdx.x < 64) {
+#pragma unroll 1
+    for (int tile = 0; tile < 2; ++tile)
+#pragma unroll 1
+      for (int ks = 0; ks < 2; ++ks) {
+        float re[8], im[8], re2[8], im2[8];
+#pragma unroll
+        for (int t = 0; t < 8; ++t) {
+          const int i = 16 * ks + 8 * h + t;
+          const int k = 32 * tile + c;
+          sincospif(-2.0f * (float)((k * i) % F) / F, &im[t], &re[t]);
+          const int j = 16 * ks + 8 * (t / 4) + 4 * h + t % 4;
+          sincospif(-2.0f * (float)((k * j) % F) / F, &im2[t], &re2[t]);
+        }
+        w1[tile][ks][l].set(re, im);
+        w2[tile][ks][l].set(re2, im2);
+      }
+  }
+  __syncthreads();
+  const long wave_id = (long)blockIdx.x * (blockDim.x >> 6) + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const long nwaves = (long)gridDim.x * (blockDim.x >> 6);
+  const int n = a.n;
+  // item it = f * nactive + idx, it advancing by nwaves (one scalar division per wave)
+  auto split_item = [&](long it, int &f, int &idx) {
+    const long q = it / a.nactive;
+    f = (int)q;
+    idx = (int)(it - q * a.nactive);
+  };
+  int stepF, stepR;
+  split_item(nwaves, stepF, stepR);
+  // lane (c, h) holds phase column j = c, rows i = 16 ks + 8h + t
+  auto load_item = [&](long it, int f, int idx, float (&ph)[16], float (&pu)[16]) {
+    const bool v = it < a.nitems;
+    const int s = v ? a.active[idx] : 0;
+    const int fr = v ? f : 0;
+    const int sy = s / a.nsubx, sx = s - (s / a.nsubx) * a.nsubx;
+    const float *p0 = a.phs + (long)fr * a.phs_stride + (long)(sy * n) * a.npup + sx * n + c;
+    const float *u0 = a.pup + (long)s * n * n + c;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      const int i = 16 * (q >> 3) + 8 * h + (q & 7);
+      const bool in = v && i < n && c < n;
+      ph[q] = in ? p0[(long)i * a.npup] : 0.f;
+      pu[q] = in ? u0[i * n] : 0.f;
+    }
+  };
+  int cf, cidx;
+  split_item(wave_id, cf, cidx);
+  float cph[16], cpu[16];
+  load_item(wave_id, cf, cidx, cph, cpu);
+  for (long it = wave_id; it < a.nitems; it += nwaves) {
+    int nf = cf + stepF, nidx = cidx + stepR;
+    if (nidx >= a.nactive) {
+      nidx -= a.nactive;
+      ++nf;
+    }
+    float nph[16], npu[16];
+    load_item(it + nwaves, nf, nidx, nph, npu);   // next item's loads overlap this item's work
+    const int frame = cf, s = a.active[cidx];
+    // ---- E = pup exp(i (phi - t)) and its hi/lo operands ----
+    COp<T8> A1[2];
+#pragma unroll
+    for (int ks = 0; ks < 2; ++ks) {
+      float er[8], ei[8];
+#pragma unroll
+      for (int t = 0; t < 8; ++t) {
+        const int i = 16 * ks + 8 * h + t;
+        const float th = cph[8 * ks + t] - a.tiltstep * (float)(i + c + 1 - n);   // float ops as the reference
+        float sn, cs;
+        dft_sincos<FAST>(th, &sn, &cs);
+        er[t] = cpu[8 * ks + t] * cs;
+        ei[t] = cpu[8 * ks + t] * sn;
+      }
+      A1[ks].set(er, ei);
+    }
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      cph[q] = nph[q];
+      cpu[q] = npu[q];
+    }
+    cf = nf;
+    cidx = nidx;
+    // reload the LDS constants where used (an opaque zero offset keeps the reads in the loop)
+    int lofs = 0;
+    asm volatile("" : "+v"(lofs));
+    // ---- per k tile: stage 1 (Z^T tile, split into stage 2's B operands), then stage 2 for
+    // both l tiles with |X|^2 binned 4x4: in-lane over u, over the quad's four lanes along k ----
+    float pix[2][2][4];   // [lt][kt][g]
+#pragma unroll
+    for (int kt = 0; kt < 2; ++kt) {
+      COp<T8> B2[2];   // [ks]
+      {
+        f16v zr = {}, zi = {};
+#pragma unroll
+        for (int ks = 0; ks < 2; ++ks) {
+          const CConst<T8> w = w1[kt][ks][l + lofs];
+          cgemm_vc(A1[ks], w, zr, zi);
+        }
+#pragma unroll
+        for (int ks = 0; ks < 2; ++ks) {
+          float br[8], bi[8];
+#pragma unroll
+          for (int q = 0; q < 8; ++q) {
+            br[q] = zr[8 * ks + q];
+            bi[q] = zi[8 * ks + q];
+          }
+          B2[ks].set(br, bi);
+        }
+      }
+#pragma unroll
+      for (int lt = 0; lt < 2; ++lt) {
+        f16v xr = {}, xi = {};
+#pragma unroll
+        for (int ks = 0; ks < 2; ++ks) {
+          const CConst<T8> w = w2[lt][ks][l + lofs];
+          cgemm_cv(w, B2[ks], xr, xi);
+        }
+#pragma unroll
+        for (int g = 0; g < 4; ++g) {
+          const int r = 4 * g;
+          float v = fmaf(xr[r], xr[r], xi[r] * xi[r]);
+          v = fmaf(xr[r + 1], xr[r + 1], fmaf(xi[r + 1], xi[r + 1], v));
+          v = fmaf(xr[r + 2], xr[r + 2], fmaf(xi[r + 2], xi[r + 2], v));
+          v = fmaf(xr[r + 3], xr[r + 3], fmaf(xi[r + 3], xi[r + 3], v));
+          v += dpp_mov<dpp::kQuadXor1>(v);
+          v += dpp_mov<dpp::kQuadXor2>(v);
+          pix[lt][kt][g] = v;
+        }
+      }
+    }
+    // ---- detector: lane (c, h) takes g = c % 4 of every tile: four distinct pixels ----
+    const int g = c & 3;
+    float val[4];
+    int pI[4], pJ[4];
+#pragma unroll
+    for (int lt = 0; lt < 2; ++lt)
+#pragma unroll
+      for (int kt = 0; kt < 2; ++kt) {
+        const int e = 2 * lt + kt;
+        float v = pix[lt][kt][0];
+        v = g == 1 ? pix[lt][kt][1] : v;
+        v = g == 2 ? pix[lt][kt][2] : v;
+        v = g == 3 ? pix[lt][kt][3] : v;
+        val[e] = v;
+        pI[e] = M - 1 - 8 * kt - (c >> 2);
+        pJ[e] = M - 1 - 8 * lt - 2 * g - h;
+      }
+    const int M2 = M * M;
+    const long pix0 = (long)s * M2;
+    const float scale = a.scale[s];
+    const float skyadd = a.skyadd[s];
+    const int st0 = (M - a.ncen) / 2, en0 = M - st0;
+    float sw = 0.f, sxw = 0.f, syw = 0.f;
+    if (!a.slow_detector) {
+      const float add = (skyadd > 0.f ? skyadd : 0.f) + (a.threshType >= 0 && a.threshType <= 2 ? a.readbg : 0.f);
+      const float thr = a.floor;
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        float v = val[e] * scale + add;
+        if (a.threshType == 1) v = v < thr ? 0.f : v;
+        else if (a.threshType == 0) v = v < thr ? 0.f : v - thr;
+        if (pI[e] >= st0 && pI[e] < en0 && pJ[e] >= st0 && pJ[e] < en0) {
+          sw += v;
+          sxw += v * pJ[e];
+          syw += v * pI[e];
+        }
+      }
+    } else {
+      // general detector in an LDS-staged runtime loop (rare options, code emitted once)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) px[e] = val[e];
+      const long noff = (long)frame * a.noise_stride + pix0;
+      const bool pois = a.poisson_on && a.poisson_on[s];
+      const bool rn = a.readnoise != 0.f && a.zread;
+      float mx = -INFINITY;
+#pragma unroll 1
+      for (int e = 0; e < 4; ++e) {
+        const int I = M - 1 - 8 * (e & 1) - (c >> 2), J = M - 1 - 8 * (e >> 1) - 2 * g - h;
+        const long p = I * M + J;
+        float v = px[e] * scale;
+        if (skyadd > 0.f) v += skyadd;
+        if (a.skyarr) v += a.skyarr[pix0 + p];
+        if (pois) v = poisson_from_normal(v, a.zpois[noff + p]);
+        if (a.threshType >= 0 && a.threshType <= 2) v += rn ? fmaf(a.readnoise, a.zread[noff + p], a.readbg) : a.readbg;
+        mx = fmaxf(mx, v);
+        px[e] = v;
+      }
+      float thr = a.floor;
+      if (a.threshType == 2) {
+        mx = wave_maximum(mx);
+        if (a.floor > 0.f && a.floor <= 1.f) thr = a.floor * fmaxf(mx, 0.f);
+      }
+      float *bout = a.bimg ? a.bimg + (long)frame * a.bimg_stride + pix0 : nullptr;
+      const float *wt = a.centWeight ? a.centWeight + (a.centWeightDim == 4 ? pix0 : 0) : nullptr;
+#pragma unroll 1
+      for (int e = 0; e < 4; ++e) {
+        const int I = M - 1 - 8 * (e & 1) - (c >> 2), J = M - 1 - 8 * (e >> 1) - 2 * g - h;
+        float v = px[e];
+        if (a.threshType == 1) v = v < thr ? 0.f : v;
+        else if (a.threshType == 0 || a.threshType == 2) v = v < thr ? 0.f : v - thr;
+        if (a.pxlPower == 2.f) v = v * v;
+        else if (a.pxlPower != 1.f && a.pxlPower != 0.f) v = powf(v, a.pxlPower);
+        if (bout) bout[I * M + J] = v;
+        if (I >= st0 && I < en0 && J >= st0 && J < en0) {
+          const float tt = wt ? v * wt[I * M + J] : v;
+          sw += tt;
+          sxw += tt * J;
+          syw += tt * I;
+        }
+      }
+    }
+    sw = wave_total(sw);
+    sxw = wave_total(sxw);
+    syw = wave_total(syw);
+    if (l == 0 && !a.no_cents) {
+      float cx = 0.f, cy = 0.f;
+      if (sw > 0.f) {
+        const float rw = __builtin_amdgcn_rcpf(sw);   // 1 ulp, inside this form's hi/lo error
+        cx = sxw * rw - (M - 1) / 2.f;
+        cy = syw * rw - (M - 1) / 2.f;
+      }
+      if (a.refCents) {
+        cx -= a.refCents[s * 2];
+        cy -= a.refCents[s * 2 + 1];
+      }
+      float *co = a.cents + (long)frame * a.cents_stride + s * 2;
+      co[0] = cx;
+      co[1] = cy;
+    }
+  }
+}
+
+bool dft32_supported(int n, int F, int nimg) { return n > 16 && n <= 32 && F == kF64 && nimg == kM64; }
+
+hipError_t launch_dft32(const FusedArgs &a, int bf16, hipStream_t st) {
+  if (a.nitems <= 0) return hipSuccess;
+  long blocks = (a.nitems + 3) / 4;
+  const long cap = 256L * 2;   // resident workgroups (two waves per SIMD), then grid-stride
+  if (blocks > cap) blocks = cap;
+  if (bf16) {
+    if (a.fast_trig) hipLaunchKernelGGL((dft32_cog_kernel<b8, true>), dim3((unsigned)blocks), dim3(256), 0, st, a);
+    else hipLaunchKernelGGL((dft32_cog_kernel<b8, false>), dim3((unsigned)blocks), dim3(256), 0, st, a);
+  } else {
+    if (a.fast_trig) hipLaunchKernelGGL((dft32_cog_kernel<h8, true>), dim3((unsigned)blocks), dim3(256), 0, st, a);
+    else hipLaunchKernelGGL((dft32_cog_kernel<h8, false>), dim3((unsigned)blocks), dim3(256), 0, st, a);
+  }
+  return hipGetLastError();
+}
+
+}  // namespace dasp

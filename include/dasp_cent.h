@@ -153,7 +153,9 @@ int dasp_cent_testcorrelation(const float *img, float *corr, int n);
 
 /* ---- extensions (no counterpart in the reference) ---------------------------------------- */
 
-/* Select the compute form (DASP_CENT_FORM_*). */
+/* Select the compute form (DASP_CENT_FORM_*). The DFT-GEMM forms cover nimg = 16 with n = 16 /
+ * fftsize = clipsize = 32 and n = 17..32 / fftsize = clipsize = 64, the fused kernels' options and
+ * no spot kernel; other geometries return DASP_CENT_EUNSUPPORTED. */
 int dasp_cent_set_form(dasp_cent_t *h, int form);
 
 /* Phase factor exp(i(phi - t)) in the fused kernels: 1 (default) = two-constant Cody-Waite

@@ -281,6 +281,53 @@ def test_dft_mfma_form_error_vs_fp64(form):
     c.free()
 
 
+DFT32_CASES = {
+    # name: (nsubx, n, correlation + parabolic fit, noise)
+    "n32_cog": (16, 32, False, False),
+    "n24_cog": (16, 24, False, False),
+    "n32_corr_fit5": (12, 32, True, False),
+    "n32_noise_thresh": (12, 32, False, True),
+}
+
+
+@pytest.mark.parametrize("form", ["fp16", "bf16"])
+@pytest.mark.parametrize("name", sorted(DFT32_CASES))
+def test_dft32_mfma_form_error_vs_fp64(name, form):
+    """The DFT-GEMM form for phase size n <= 32, fftsize 64, nimg 16 (4x4 binning) against FP64:
+    noiseless centroids to 1e-4 px (FP16 split) / RMS 1e-3 px (BF16 split); the images the
+    correlation and fit stage reads within 5e-5 / 5e-4 of the image maximum."""
+    from dasp_amd.cmod import cent
+    nsubx, n, corr, noise = DFT32_CASES[name]
+    say("case DFT32 %s %s" % (name, form))
+    wgeo = W.WFS("dft32", nsubx, n, 64, 16, 8.0, 0.2, 0.15, 25.0)
+    kw = dict(obs=0.2, phs=wgeo.phase(), form={"fp16": cent.FORM_DFT_FP16, "bf16": cent.FORM_DFT_BF16}[form])
+    if corr:
+        kw.update(correlationCentroiding=1, corrPattern=W.reference_images(wgeo), parabolicFit=5)
+    if noise:
+        kw.update(sig=1000.0, readnoise=0.5, addPoisson=1, noiseFloor=0.1, threshType=2, seed=9)
+    c = Case(nsubx, n, 64, 16, **kw)
+    oc, ob, _ = c.run_both()
+    act = _active(c)
+    b = c.bimg.reshape(-1, 16, 16)[act]
+    if noise:
+        diffpix = np.abs(b - ob[act]) > 1e-3 * np.maximum(1, np.abs(ob[act]))
+        say("noisy pixels differing: %d of %d" % (diffpix.sum(), diffpix.size))
+        assert diffpix.sum() <= max(3, 1e-3 * diffpix.size)
+        clean = ~diffpix.reshape(diffpix.shape[0], -1).any(axis=1)
+        d = np.abs(c.cents.reshape(-1, 2)[act] - oc[act])[clean]
+        assert np.sqrt((d ** 2).mean()) < 1e-3
+        c.free()
+        return
+    d = report("dft32-%s-%s" % (name, form), c.cents, oc, act)
+    assert np.sqrt((d ** 2).mean()) < 1e-3
+    if form == "fp16":
+        assert d.max() < 1e-4
+    tol = 5e-5 if form == "fp16" else 5e-4
+    np.testing.assert_allclose(b, ob[act], rtol=0, atol=tol * ob[act].max())
+    assert np.all(c.cents.reshape(-1, 2)[~act] == 0)
+    c.free()
+
+
 @pytest.mark.parametrize("m", [45, 83])
 def test_interpolated_phase_grid(m):
     """Phase on an m x m grid (not nsubx*n = 64): interpolatePhase (bicubic) onto the 64 x 64
