@@ -510,8 +510,7 @@ static void fused_args(dasp_cent *c, const float *d_phs, long phs_stride, int nf
   fa.bimg = d_bimg;
   fa.bimg_stride = (long)c->nsubaps * c->nimg * c->nimg;
   fa.slow_detector = (d_zpois && need_pois(c)) || (d_zread && need_read(c)) || c->haveSkyArr ||
-                     c->threshType == 2 || (c->pxlPower != 1.f && c->pxlPower != 0.f) || c->haveCentWeight ||
-                     d_bimg != nullptr;
+                     c->threshType == 2 || (c->pxlPower != 1.f && c->pxlPower != 0.f) || c->haveCentWeight;
   fa.otfk = c->spotDim ? c->d_otfk.as<float2>() : nullptr;
   fa.otf_per_subap = c->spotDim == 4;
 }

@@ -56,6 +56,26 @@ __global__ __launch_bounds__(256) void dft16_cog_kernel(FusedArgs a_) {
   const long nwaves = (long)gridDim.x * (blockDim.x >> 6);
   const int n = a.n;
   const int jj = c & 15;
+  // fast-detector constants of this lane: its detector row (lane pair k = c, c+1: hll rows F-1-k,
+  // F-2-k), its four pixel columns (lane parity p takes q = 4p..4p+3; l pair starts at
+  // l0 = 8(q/2) + 4h + 2(q%2), J = (F-2-l0)/2) and their CoG weights inside the ncen window
+  int pJ[4];
+  float wc[4], wj[4];
+  float fI;
+  {
+    const int st0 = (kM - a.ncen) / 2, en0 = kM - st0;
+    const int I = (kF - 2 - (c & ~1)) >> 1;
+    fI = (float)I;
+#pragma unroll
+    for (int qq = 0; qq < 4; ++qq) {
+      const int q = 4 * (c & 1) + qq;
+      const int J = (kF - 2 - (8 * (q >> 1) + 4 * h + 2 * (q & 1))) >> 1;
+      const bool in = I >= st0 && I < en0 && J >= st0 && J < en0;
+      pJ[qq] = J;
+      wc[qq] = in ? 1.f : 0.f;
+      wj[qq] = in ? (float)J : 0.f;
+    }
+  }
   // frame and active index of a pair's first item (the pair index is wave-uniform: one scalar
   // division per pair)
   auto pair_ref = [&](long pr, int &f0, int &idx0) {
@@ -170,20 +190,17 @@ __global__ __launch_bounds__(256) void dft16_cog_kernel(FusedArgs a_) {
       if (!a.slow_detector) {
         const float add = (skyadd > 0.f ? skyadd : 0.f) + (a.threshType >= 0 && a.threshType <= 2 ? a.readbg : 0.f);
         const float thr = a.floor;
+        float *bout = a.bimg ? a.bimg + (long)frame * a.bimg_stride + pix0 : nullptr;
 #pragma unroll
         for (int qq = 0; qq < 4; ++qq) {
-          // l pair starts at l0 = 8(q/2) + 4h + 2(q%2): detector column J = (F-2-l0)/2
-          const int q = 4 * par + qq;
-          const int J = (kF - 2 - (8 * (q >> 1) + 4 * h + 2 * (q & 1))) >> 1;
-          float v = (par ? pix[4 + qq] : pix[qq]) * scale + add;
+          float v = fmaf(par ? pix[4 + qq] : pix[qq], scale, 0.f) + add;
           if (a.threshType == 1) v = v < thr ? 0.f : v;
-          else if (a.threshType == 0) v = v < thr ? 0.f : v - thr;
-          if (I >= st0 && I < en0 && J >= st0 && J < en0) {
-            sw += v;
-            sxw += v * J;
-            syw += v * I;
-          }
+          else if (a.threshType == 0) v = fmaxf(v - thr, 0.f);   // = (v < thr ? 0 : v - thr)
+          if (bout) bout[I * kM + pJ[qq]] = v;
+          sw = fmaf(wc[qq], v, sw);
+          sxw = fmaf(wj[qq], v, sxw);
         }
+        syw = fI * sw;   // the lane's pixels share one detector row
       } else {
         const bool owner = par == 0;
 #pragma unroll

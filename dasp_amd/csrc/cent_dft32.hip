@@ -66,19 +66,35 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void d
   int stepF, stepR;
   split_item(nwaves, stepF, stepR);
   // lane (c, h) holds phase column j = c, rows i = 16 ks + 8h + t
+  // Loads are unconditional (no per-element branches): past the last item the wave reads item 0
+  // (never used), and for n < 32 the padding rows/columns read a clamped in-window address with
+  // the pupil factor forced to 0, so E is exactly 0 there.
   auto load_item = [&](long it, int f, int idx, float (&ph)[16], float (&pu)[16]) {
-    const bool v = it < a.nitems;
+    const bool v = it < a.nitems;   // wave-uniform
     const int s = v ? a.active[idx] : 0;
     const int fr = v ? f : 0;
     const int sy = s / a.nsubx, sx = s - (s / a.nsubx) * a.nsubx;
-    const float *p0 = a.phs + (long)fr * a.phs_stride + (long)(sy * n) * a.npup + sx * n + c;
-    const float *u0 = a.pup + (long)s * n * n + c;
+    const float *p0 = a.phs + (long)fr * a.phs_stride + (long)(sy * n) * a.npup + sx * n;
+    const float *u0 = a.pup + (long)s * n * n;
+    if (n == 32) {
+      // wave-uniform row bases (scalar registers) plus one 32-bit lane offset
+      const int po = 8 * h * a.npup + c, uo = 8 * h * 32 + c;
 #pragma unroll
-    for (int q = 0; q < 16; ++q) {
-      const int i = 16 * (q >> 3) + 8 * h + (q & 7);
-      const bool in = v && i < n && c < n;
-      ph[q] = in ? p0[(long)i * a.npup] : 0.f;
-      pu[q] = in ? u0[i * n] : 0.f;
+      for (int q = 0; q < 16; ++q) {
+        const int r = 16 * (q >> 3) + (q & 7);
+        ph[q] = (p0 + (long)r * a.npup)[po];
+        pu[q] = (u0 + r * 32)[uo];
+      }
+    } else {
+      const int cc = c < n ? c : n - 1;
+#pragma unroll
+      for (int q = 0; q < 16; ++q) {
+        const int i = 16 * (q >> 3) + 8 * h + (q & 7);
+        const int ii = i < n ? i : n - 1;
+        ph[q] = p0[ii * a.npup + cc];
+        const float u = u0[ii * n + cc];
+        pu[q] = (i < n && c < n) ? u : 0.f;
+      }
     }
   };
   int cf, cidx;
@@ -91,8 +107,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void d
       nidx -= a.nactive;
       ++nf;
     }
-    float nph[16], npu[16];
-    load_item(it + nwaves, nf, nidx, nph, npu);   // next item's loads overlap this item's work
     const int frame = cf, s = a.active[cidx];
     // ---- E = pup exp(i (phi - t)) and its hi/lo operands ----
     COp<T8> A1[2];
@@ -110,19 +124,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void d
       }
       A1[ks].set(er, ei);
     }
-#pragma unroll
-    for (int q = 0; q < 16; ++q) {
-      cph[q] = nph[q];
-      cpu[q] = npu[q];
-    }
-    cf = nf;
-    cidx = nidx;
     // reload the LDS constants where used (an opaque zero offset keeps the reads in the loop)
     int lofs = 0;
     asm volatile("" : "+v"(lofs));
     // ---- per k tile: stage 1 (Z^T tile, split into stage 2's B operands), then stage 2 for
     // both l tiles with |X|^2 binned 4x4: in-lane over u, over the quad's four lanes along k ----
-    float pix[2][2][4];   // [lt][kt][g]
+    float pix[2][2];   // [lt][kt]: this lane's pixel of the tile
+    const bool b0 = (c & 1) != 0, b1 = (c & 2) != 0;
 #pragma unroll
     for (int kt = 0; kt < 2; ++kt) {
       COp<T8> B2[2];   // [ks]
@@ -144,6 +152,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void d
           B2[ks].set(br, bi);
         }
       }
+      // the next item's loads, issued once the stage-1 operands are dead (their registers are
+      // not held across the whole item) and overlapping the stage-2 products
+      if (kt == 1) load_item(it + nwaves, nf, nidx, cph, cpu);
 #pragma unroll
       for (int lt = 0; lt < 2; ++lt) {
         f16v xr = {}, xi = {};
@@ -152,17 +163,23 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void d
           const CConst<T8> w = w2[lt][ks][l + lofs];
           cgemm_cv(w, B2[ks], xr, xi);
         }
+        float v[4];
 #pragma unroll
         for (int g = 0; g < 4; ++g) {
           const int r = 4 * g;
-          float v = fmaf(xr[r], xr[r], xi[r] * xi[r]);
-          v = fmaf(xr[r + 1], xr[r + 1], fmaf(xi[r + 1], xi[r + 1], v));
-          v = fmaf(xr[r + 2], xr[r + 2], fmaf(xi[r + 2], xi[r + 2], v));
-          v = fmaf(xr[r + 3], xr[r + 3], fmaf(xi[r + 3], xi[r + 3], v));
-          v += dpp_mov<dpp::kQuadXor1>(v);
-          v += dpp_mov<dpp::kQuadXor2>(v);
-          pix[lt][kt][g] = v;
+          float t = fmaf(xr[r], xr[r], xi[r] * xi[r]);
+          t = fmaf(xr[r + 1], xr[r + 1], fmaf(xi[r + 1], xi[r + 1], t));
+          t = fmaf(xr[r + 2], xr[r + 2], fmaf(xi[r + 2], xi[r + 2], t));
+          v[g] = fmaf(xr[r + 3], xr[r + 3], fmaf(xi[r + 3], xi[r + 3], t));
         }
+        // reduce-scatter over the quad (its four lanes are four k of one detector row): quad
+        // lane p ends with the total of pixel column group g = p. Step 1 exchanges with lane
+        // p ^ 2 and keeps g in {2 b1, 2 b1 + 1}; step 2 exchanges with p ^ 1 and keeps g = p.
+        const float k0 = b1 ? v[2] : v[0], k1 = b1 ? v[3] : v[1];
+        const float s0 = b1 ? v[0] : v[2], s1 = b1 ? v[1] : v[3];
+        const float w0 = k0 + dpp_mov<dpp::kQuadXor2>(s0);
+        const float w1 = k1 + dpp_mov<dpp::kQuadXor2>(s1);
+        pix[lt][kt] = (b0 ? w1 : w0) + dpp_mov<dpp::kQuadXor1>(b0 ? w0 : w1);
       }
     }
     // ---- detector: lane (c, h) takes g = c % 4 of every tile: four distinct pixels ----
@@ -174,11 +191,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void d
 #pragma unroll
       for (int kt = 0; kt < 2; ++kt) {
         const int e = 2 * lt + kt;
-        float v = pix[lt][kt][0];
-        v = g == 1 ? pix[lt][kt][1] : v;
-        v = g == 2 ? pix[lt][kt][2] : v;
-        v = g == 3 ? pix[lt][kt][3] : v;
-        val[e] = v;
+        val[e] = pix[lt][kt];
         pI[e] = M - 1 - 8 * kt - (c >> 2);
         pJ[e] = M - 1 - 8 * lt - 2 * g - h;
       }
@@ -187,15 +200,19 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void d
     const float scale = a.scale[s];
     const float skyadd = a.skyadd[s];
     const int st0 = (M - a.ncen) / 2, en0 = M - st0;
+    cf = nf;
+    cidx = nidx;
     float sw = 0.f, sxw = 0.f, syw = 0.f;
     if (!a.slow_detector) {
       const float add = (skyadd > 0.f ? skyadd : 0.f) + (a.threshType >= 0 && a.threshType <= 2 ? a.readbg : 0.f);
       const float thr = a.floor;
+      float *bout = a.bimg ? a.bimg + (long)frame * a.bimg_stride + pix0 : nullptr;
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
-        float v = val[e] * scale + add;
+        float v = fmaf(val[e], scale, 0.f) + add;
         if (a.threshType == 1) v = v < thr ? 0.f : v;
         else if (a.threshType == 0) v = v < thr ? 0.f : v - thr;
+        if (bout) bout[pI[e] * M + pJ[e]] = v;
         if (pI[e] >= st0 && pI[e] < en0 && pJ[e] >= st0 && pJ[e] < en0) {
           sw += v;
           sxw += v * pJ[e];

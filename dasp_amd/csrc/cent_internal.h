@@ -117,7 +117,7 @@ struct FusedArgs {
   const float *tilt;     // (n, n)
   float tiltstep;        // the reference's float tmp = pi/F*(F+1): tilt[i][j] = tmp*(i+j+1-n)
   int fast_trig;         // 1: Cody-Waite reduction + hardware sin/cos instead of sincosf
-  int slow_detector;     // any of noise, sky array, relative floor, power, weights, image output
+  int slow_detector;     // any of noise, sky array, relative floor, power, weights
   const int *active;
   int nactive;
   long nitems;

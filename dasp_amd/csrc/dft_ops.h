@@ -27,12 +27,16 @@ struct Split<h8> {
     typedef unsigned u4 __attribute__((ext_vector_type(4)));
     typedef _Float16 h2 __attribute__((ext_vector_type(2)));
     u4 hw, lw;
+    // an opaque 1.0 (one SGPR): fma(x, 1, -hi) would otherwise be folded into x - hi, which is
+    // then emitted as two conversions, a subtraction and a repack instead of one v_fma_mix
+    float one = 1.0f;
+    asm("" : "+s"(one));
 #pragma unroll
     for (int t = 0; t < 8; t += 2) {
       const h2 hp = __builtin_bit_cast(h2, __builtin_amdgcn_cvt_pkrtz(x[t], x[t + 1]));
       h2 lp;
-      lp[0] = (_Float16)__builtin_fmaf((float)hp[0], -1.0f, x[t]);
-      lp[1] = (_Float16)__builtin_fmaf((float)hp[1], -1.0f, x[t + 1]);
+      lp[0] = (_Float16)__builtin_fmaf(x[t], one, -(float)hp[0]);
+      lp[1] = (_Float16)__builtin_fmaf(x[t + 1], one, -(float)hp[1]);
       hw[t / 2] = __builtin_bit_cast(unsigned, hp);
       lw[t / 2] = __builtin_bit_cast(unsigned, lp);
     }

@@ -290,9 +290,10 @@ __device__ __forceinline__ void fused_item(const FusedArgs &a, const ItemRef &it
   const int st0 = (NIMG - a.ncen) / 2, en0 = NIMG - st0;
   float sw = 0.f, sxw = 0.f, syw = 0.f;
   if (!a.slow_detector) {
-    // no noise, sky array, relative floor, power, weights or image output: all inline
+    // no noise, sky array, relative floor, power or weights: all inline
     const float add = (skyadd > 0.f ? skyadd : 0.f) + (a.threshType >= 0 && a.threshType <= 2 ? a.readbg : 0.f);
     const float thr = a.floor;
+    float *bout = a.bimg && owner ? a.bimg + (long)frame * a.bimg_stride + pix0 : nullptr;
     static_for<0, RB>([&](auto ibb) {
       constexpr int ib = decltype(ibb)::value;
       const int I = det_row(ib);
@@ -301,6 +302,7 @@ __device__ __forceinline__ void fused_item(const FusedArgs &a, const ItemRef &it
         float v = fmaf(acc[ib][jb], scale, 0.f) + add;
         if (a.threshType == 1) v = v < thr ? 0.f : v;
         else if (a.threshType == 0) v = v < thr ? 0.f : v - thr;
+        if (bout) bout[I * NIMG + jb] = v;
         if (owner && I >= st0 && I < en0 && jb >= st0 && jb < en0) {
           sw += v;
           sxw += v * jb;
