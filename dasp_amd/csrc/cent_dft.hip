@@ -95,20 +95,26 @@ __global__ __launch_bounds__(256) void dft16_cog_kernel(FusedArgs a_) {
   // stage-1 operand of a pair: lane holds E_s[i][j], s = c/16, j = c%16, i = 8h + t
   auto load_pair = [&](long pr, int f0, int idx0, float (&ph8)[8], float (&pu8)[8]) {
     const int u = c >> 4;
-    int fr, idx;
-    next_item(f0, idx0, u, fr, idx);
-    const bool v = pr < npairs && 2 * pr + u < a.nitems;
-    const int sa = v ? a.active[idx] : 0;
-    fr = v ? fr : 0;
-    const int sy = sa / a.nsubx, sx = sa - (sa / a.nsubx) * a.nsubx;
-    const float *ph = a.phs + (long)fr * a.phs_stride + (long)(sy * n) * a.npup + sx * n + jj;
-    const float *pu = a.pup + (long)sa * n * n + jj;
+    // both items' sub-aperture indices and window offsets in scalar registers (scalar-cache
+    // reads, scalar division), then the lane's item by a select. Loads are unconditional: an
+    // item past the end reads item 0 (its results are never used).
+    int fA, iA, fB, iB;
+    next_item(f0, idx0, 0, fA, iA);
+    next_item(f0, idx0, 1, fB, iB);
+    const bool vA = pr < npairs, vB = vA && 2 * pr + 1 < a.nitems;
+    const int sA = vA ? scalar_read(a.active, iA) : 0, sB = vB ? scalar_read(a.active, iB) : 0;
+    auto window = [&](int sa, int fr) {
+      const int sy = sa / a.nsubx, sx = sa - sy * a.nsubx;
+      return (long)fr * a.phs_stride + (long)(sy * kN) * a.npup + sx * kN;
+    };
+    const long wA = window(sA, vA ? fA : 0), wB = window(sB, vB ? fB : 0);
+    const float *ph = a.phs + (u ? wB : wA) + jj;
+    const float *pu = a.pup + (long)(u ? sB : sA) * (kN * kN) + jj;
 #pragma unroll
     for (int t = 0; t < 8; ++t) {
       const int i = 8 * h + t;
-      const bool in = v && i < n && jj < n;
-      ph8[t] = in ? ph[(long)i * a.npup] : 0.f;
-      pu8[t] = in ? pu[i * n] : 0.f;
+      ph8[t] = ph[(long)i * a.npup];
+      pu8[t] = pu[i * kN];
     }
   };
   float cph[8], cpu[8];
@@ -155,7 +161,7 @@ __global__ __launch_bounds__(256) void dft16_cog_kernel(FusedArgs a_) {
       if (2 * pr + sub >= a.nitems) break;   // wave-uniform
       int frame, sidx;
       next_item(pf0, pidx0, sub, frame, sidx);
-      const int s = a.active[sidx];
+      const int s = scalar_read(a.active, sidx);
       // ---- stage 2: B2 = Z_s^T from accumulator registers 8*sub .. 8*sub+7 ----
       float br[8], bi[8];
 #pragma unroll
@@ -183,8 +189,8 @@ __global__ __launch_bounds__(256) void dft16_cog_kernel(FusedArgs a_) {
       const int I = (kF - 2 - (c & ~1)) >> 1;
       const int M2 = kM * kM;
       const long pix0 = (long)s * M2;
-      const float scale = a.scale[s];
-      const float skyadd = a.skyadd[s];
+      const float scale = scalar_read(a.scale, s);
+      const float skyadd = scalar_read(a.skyadd, s);
       const int st0 = (kM - a.ncen) / 2, en0 = kM - st0;
       float sw = 0.f, sxw = 0.f, syw = 0.f;
       if (!a.slow_detector) {
@@ -262,8 +268,8 @@ __global__ __launch_bounds__(256) void dft16_cog_kernel(FusedArgs a_) {
           cy = syw * rw - (kM - 1) / 2.f;
         }
         if (a.refCents) {
-          cx -= a.refCents[s * 2];
-          cy -= a.refCents[s * 2 + 1];
+          cx -= scalar_read(a.refCents, s * 2);
+          cy -= scalar_read(a.refCents, s * 2 + 1);
         }
         float *co = a.cents + (long)frame * a.cents_stride + s * 2;
         co[0] = cx;

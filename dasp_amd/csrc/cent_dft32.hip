@@ -23,13 +23,14 @@ namespace {
 constexpr int kF64 = 64, kM64 = 16;
 }
 
-template <typename T8, bool FAST>
+
+template <typename T8, bool FAST, bool FULL>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void dft32_cog_kernel(FusedArgs a_) {
   typedef const __attribute__((address_space(4))) FusedArgs KernArgs;
   const FusedArgs &a = *(const FusedArgs *)(KernArgs *)__builtin_amdgcn_kernarg_segment_ptr();
   constexpr int F = kF64, M = kM64;
-  __shared__ CConst<T8> w1[2][2][64];   // [kt][ks]: W[32 kt + c][16 ks + 8h + t]
-  __shared__ CConst<T8> w2[2][2][64];   // [lt][ks]: W[32 lt + c][16 ks + 8(t/4) + 4h + t%4]
+  __shared__ CConstLds<T8> w1[2][2];   // [kt][ks]: W[32 kt + c][16 ks + 8h + t]
+  __shared__ CConstLds<T8> w2[2][2];   // [lt][ks]: W[32 lt + c][16 ks + 8(t/4) + 4h + t%4]
   __shared__ float pxbuf[4][64 * 4];
   const int l = threadIdx.x & 63;
   const int h = l >> 5;
@@ -49,8 +50,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void d
           const int j = 16 * ks + 8 * (t / 4) + 4 * h + t % 4;
           sincospif(-2.0f * (float)((k * j) % F) / F, &im2[t], &re2[t]);
         }
-        w1[tile][ks][l].set(re, im);
-        w2[tile][ks][l].set(re2, im2);
+        CConst<T8> v1, v2;
+        v1.set(re, im);
+        v2.set(re2, im2);
+        w1[tile][ks].put(l, v1);
+        w2[tile][ks].put(l, v2);
       }
   }
   __syncthreads();
@@ -71,12 +75,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void d
   // the pupil factor forced to 0, so E is exactly 0 there.
   auto load_item = [&](long it, int f, int idx, float (&ph)[16], float (&pu)[16]) {
     const bool v = it < a.nitems;   // wave-uniform
-    const int s = v ? a.active[idx] : 0;
+    const int s = v ? scalar_read(a.active, idx) : 0;
     const int fr = v ? f : 0;
     const int sy = s / a.nsubx, sx = s - (s / a.nsubx) * a.nsubx;
     const float *p0 = a.phs + (long)fr * a.phs_stride + (long)(sy * n) * a.npup + sx * n;
     const float *u0 = a.pup + (long)s * n * n;
-    if (n == 32) {
+    if constexpr (FULL) {
       // wave-uniform row bases (scalar registers) plus one 32-bit lane offset
       const int po = 8 * h * a.npup + c, uo = 8 * h * 32 + c;
 #pragma unroll
@@ -86,14 +90,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void d
         pu[q] = (u0 + r * 32)[uo];
       }
     } else {
+      // the zero factor is a product, not a select, so that the load is not made conditional
       const int cc = c < n ? c : n - 1;
 #pragma unroll
       for (int q = 0; q < 16; ++q) {
         const int i = 16 * (q >> 3) + 8 * h + (q & 7);
         const int ii = i < n ? i : n - 1;
         ph[q] = p0[ii * a.npup + cc];
-        const float u = u0[ii * n + cc];
-        pu[q] = (i < n && c < n) ? u : 0.f;
+        pu[q] = u0[ii * n + cc] * ((i < n && c < n) ? 1.f : 0.f);
       }
     }
   };
@@ -107,7 +111,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void d
       nidx -= a.nactive;
       ++nf;
     }
-    const int frame = cf, s = a.active[cidx];
+    const int frame = cf, s = scalar_read(a.active, cidx);
     // ---- E = pup exp(i (phi - t)) and its hi/lo operands ----
     COp<T8> A1[2];
 #pragma unroll
@@ -138,7 +142,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void d
         f16v zr = {}, zi = {};
 #pragma unroll
         for (int ks = 0; ks < 2; ++ks) {
-          const CConst<T8> w = w1[kt][ks][l + lofs];
+          const CConst<T8> w = w1[kt][ks].get(l + lofs);
           cgemm_vc(A1[ks], w, zr, zi);
         }
 #pragma unroll
@@ -160,7 +164,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void d
         f16v xr = {}, xi = {};
 #pragma unroll
         for (int ks = 0; ks < 2; ++ks) {
-          const CConst<T8> w = w2[lt][ks][l + lofs];
+          const CConst<T8> w = w2[lt][ks].get(l + lofs);
           cgemm_cv(w, B2[ks], xr, xi);
         }
         float v[4];
@@ -197,8 +201,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void d
       }
     const int M2 = M * M;
     const long pix0 = (long)s * M2;
-    const float scale = a.scale[s];
-    const float skyadd = a.skyadd[s];
+    const float scale = scalar_read(a.scale, s);
+    const float skyadd = scalar_read(a.skyadd, s);
     const int st0 = (M - a.ncen) / 2, en0 = M - st0;
     cf = nf;
     cidx = nidx;
@@ -274,8 +278,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void d
         cy = syw * rw - (M - 1) / 2.f;
       }
       if (a.refCents) {
-        cx -= a.refCents[s * 2];
-        cy -= a.refCents[s * 2 + 1];
+        cx -= scalar_read(a.refCents, s * 2);
+        cy -= scalar_read(a.refCents, s * 2 + 1);
       }
       float *co = a.cents + (long)frame * a.cents_stride + s * 2;
       co[0] = cx;
@@ -286,17 +290,25 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void d
 
 bool dft32_supported(int n, int F, int nimg) { return n > 16 && n <= 32 && F == kF64 && nimg == kM64; }
 
+template <typename T8, bool FAST>
+static void launch_dft32_t(const FusedArgs &a, long blocks, hipStream_t st) {
+  if (a.n == 32) hipLaunchKernelGGL((dft32_cog_kernel<T8, FAST, true>), dim3((unsigned)blocks), dim3(256), 0, st, a);
+  else hipLaunchKernelGGL((dft32_cog_kernel<T8, FAST, false>), dim3((unsigned)blocks), dim3(256), 0, st, a);
+}
+
 hipError_t launch_dft32(const FusedArgs &a, int bf16, hipStream_t st) {
   if (a.nitems <= 0) return hipSuccess;
   long blocks = (a.nitems + 3) / 4;
-  const long cap = 256L * 2;   // resident workgroups (two waves per SIMD), then grid-stride
+  // resident workgroups (two waves per SIMD), then grid-stride; three waves per SIMD (168 VGPRs)
+  // measured no faster
+  const long cap = 256L * 2;
   if (blocks > cap) blocks = cap;
   if (bf16) {
-    if (a.fast_trig) hipLaunchKernelGGL((dft32_cog_kernel<b8, true>), dim3((unsigned)blocks), dim3(256), 0, st, a);
-    else hipLaunchKernelGGL((dft32_cog_kernel<b8, false>), dim3((unsigned)blocks), dim3(256), 0, st, a);
+    if (a.fast_trig) launch_dft32_t<b8, true>(a, blocks, st);
+    else launch_dft32_t<b8, false>(a, blocks, st);
   } else {
-    if (a.fast_trig) hipLaunchKernelGGL((dft32_cog_kernel<h8, true>), dim3((unsigned)blocks), dim3(256), 0, st, a);
-    else hipLaunchKernelGGL((dft32_cog_kernel<h8, false>), dim3((unsigned)blocks), dim3(256), 0, st, a);
+    if (a.fast_trig) launch_dft32_t<h8, true>(a, blocks, st);
+    else launch_dft32_t<h8, false>(a, blocks, st);
   }
   return hipGetLastError();
 }

@@ -6,6 +6,15 @@ namespace dasp {
 
 constexpr int kMaxSide = 64;  // largest psfsize/clipsize/nimg/corrsize handled on the device
 
+// Read of a read-only input at a wave-uniform index through the scalar cache (constant address
+// space). Such reads are counted by the scalar-memory counter, so waiting for them never waits
+// for the wave's outstanding vector loads (prefetches of the next item).
+template <typename T>
+__device__ __forceinline__ T scalar_read(const T *p, long i) {
+  typedef const __attribute__((address_space(4))) T CT;
+  return ((CT *)p)[i];
+}
+
 // Image formation: phase -> |FFT|^2 placed as the reference's hll (cmod/src/centmodule.c:473-509).
 struct FormArgs {
   const float *phs;      // frame 0 phase (npup x npup)

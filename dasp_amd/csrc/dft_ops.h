@@ -89,6 +89,32 @@ struct CConst {
   }
 };
 
+// A wave's 64 lanes of one constant operand in LDS, member by member (structure of arrays): a
+// lane's 16-byte member sits 16 bytes after its neighbour's, so each ds_read_b128 is free of
+// bank conflicts (an array of 96-byte CConst records puts every fourth lane on the same banks).
+template <typename T8>
+struct CConstLds {
+  T8 rh[64], rl[64], ih[64], il[64], nih[64], nil[64];
+  __device__ __forceinline__ void put(int l, const CConst<T8> &c) {
+    rh[l] = c.rh;
+    rl[l] = c.rl;
+    ih[l] = c.ih;
+    il[l] = c.il;
+    nih[l] = c.nih;
+    nil[l] = c.nil;
+  }
+  __device__ __forceinline__ CConst<T8> get(int l) const {
+    CConst<T8> c;
+    c.rh = rh[l];
+    c.rl = rl[l];
+    c.ih = ih[l];
+    c.il = il[l];
+    c.nih = nih[l];
+    c.nil = nil[l];
+    return c;
+  }
+};
+
 // (A.re + i A.im)(B.re + i B.im) accumulated into (cr, ci), three products per real product.
 // Stage 1: A varies per item, B = W^T is constant (negation on B).
 template <typename T8>
