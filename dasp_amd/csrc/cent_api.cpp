@@ -549,6 +549,8 @@ static int launch_phase(dasp_cent *c, const float *d_phs, long phs_stride, int n
     if (trc) return trc;
     if (use_dft(c)) HIPCHK(launch_dft(fa, c->form == DASP_CENT_FORM_DFT_BF16, st));
     else HIPCHK(launch_fused(fa, np, c->F, c->spotDim != 0, st));
+    trc = timing_mark(c, st);   // the timed interval is the image kernel alone
+    if (trc) return trc;
     if (stage) {
       CorrArgs ca{};
       const bool corr = c->correlationCentroiding == 1;
@@ -578,8 +580,6 @@ static int launch_phase(dasp_cent *c, const float *d_phs, long phs_stride, int n
       ca.corr_stride = (long)c->nsubaps * N * N;
       HIPCHK(launch_corr_fit(ca, N, corr, st));
     }
-    trc = timing_mark(c, st);
-    if (trc) return trc;
   } else if (nitems > 0) {
     const long hll_elems = nitems * (long)c->padded * c->padded;
     HIPCHK(c->d_hll.ensure(hll_elems * sizeof(float)));

@@ -16,6 +16,8 @@
 
 namespace dasp {
 
+constexpr int kXcds = 8;
+
 namespace {
 
 __device__ __forceinline__ void wave_sync() {
@@ -54,14 +56,22 @@ __global__ __launch_bounds__(256) void corr_fit_kernel(CorrArgs a) {
   const int r = lane - sl * N;
   cf *T = &stage[w][sl * N * TS];
   float *Tf = reinterpret_cast<float *>(T);
-  const long ngroups = (a.nitems + IPW - 1) / IPW;
-  for (long g = (long)blockIdx.x * WAVES + w; g < ngroups; g += (long)gridDim.x * WAVES) {
-    const long item = g * IPW + sl;
-    const bool valid = item < a.nitems;
+  // XCD-aware order: workgroup b runs on XCD b % 8, and XCD x takes the x-th eighth of the active
+  // sub-apertures for every frame, so that its L2 keeps that slice's pattern spectra (read once
+  // per item otherwise from HBM). Groups of IPW neighbouring sub-apertures of the slice, frame-major.
+  const int xcd = blockIdx.x % kXcds;
+  const int s0 = (int)((long)a.nactive * xcd / kXcds);
+  const int cnt = (int)((long)a.nactive * (xcd + 1) / kXcds) - s0;
+  const int gpf = (cnt + IPW - 1) / IPW;                               // groups per frame
+  const long ngroups = cnt > 0 ? (a.nitems / a.nactive) * gpf : 0;    // this XCD's groups
+  for (long g = (long)(blockIdx.x / kXcds) * WAVES + w; g < ngroups; g += (long)(gridDim.x / kXcds) * WAVES) {
+    const long gf = g / gpf;
+    const int rr = (int)(g - gf * gpf) * IPW + sl;
+    const bool valid = rr < cnt;
     int frame = 0, s = 0;
     if (valid) {
-      frame = (int)(item / a.nactive);
-      s = a.active[item - (long)frame * a.nactive];
+      frame = (int)gf;
+      s = a.active[s0 + rr];
     }
     const int M = a.nimg;
     const float *img = a.img + (long)frame * a.img_stride + (long)s * M * M;
@@ -253,6 +263,7 @@ hipError_t launch_corr_fit(const CorrArgs &a, int n, bool corr, hipStream_t st) 
   long blocks = (a.nitems + ipb - 1) / ipb;
   if (blocks <= 0) return hipSuccess;
   if (blocks > 2048) blocks = 2048;
+  blocks = (blocks + kXcds - 1) / kXcds * kXcds;   // every XCD gets the same number of workgroups
 #define DASP_CORR_CASE(NN)                                                                                  \
   if (n == NN) {                                                                                          \
     if (corr) hipLaunchKernelGGL((corr_fit_kernel<NN, true>), dim3((unsigned)blocks), dim3(256), 0, st, a);  \

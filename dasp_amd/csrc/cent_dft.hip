@@ -24,6 +24,7 @@
 namespace dasp {
 
 constexpr int kN = 16, kF = 32, kM = 16;   // phase size, fft size, detector size (bin 2)
+constexpr int kXcds = 8;
 
 template <typename T8, bool FAST>
 __global__ __launch_bounds__(256) void dft16_cog_kernel(FusedArgs a_) {
@@ -51,9 +52,18 @@ __global__ __launch_bounds__(256) void dft16_cog_kernel(FusedArgs a_) {
     B1.set(re, im);
     A2.set(re2, im2);
   }
-  const long npairs = (a.nitems + 1) / 2;
-  const long wave_id = (long)blockIdx.x * (blockDim.x >> 6) + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const long nwaves = (long)gridDim.x * (blockDim.x >> 6);
+  // XCD-aware pair order: workgroup b runs on XCD b % 8, and XCD x takes the x-th eighth of the
+  // active sub-apertures for every frame (its L2 keeps that slice's pupil). A pair is two
+  // neighbouring sub-apertures of the slice in one frame (the slice's last pair of a frame may
+  // hold one); the XCD's pairs are taken frame-major with a grid stride.
+  const int xcd = blockIdx.x % kXcds;
+  const int s0 = (int)((long)a.nactive * xcd / kXcds);
+  const int cnt = (int)((long)a.nactive * (xcd + 1) / kXcds) - s0;
+  const int pp = (cnt + 1) / 2;                                   // pairs per frame
+  const long npairs = cnt > 0 ? (a.nitems / a.nactive) * pp : 0;  // this XCD's pairs
+  const long wave_id = (long)(blockIdx.x / kXcds) * (blockDim.x >> 6) + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const long nwaves = (long)(gridDim.x / kXcds) * (blockDim.x >> 6);
+  if (wave_id >= npairs) return;   // wave-uniform; no barrier follows
   const int n = a.n;
   const int jj = c & 15;
   // fast-detector constants of this lane: its detector row (lane pair k = c, c+1: hll rows F-1-k,
@@ -76,33 +86,23 @@ __global__ __launch_bounds__(256) void dft16_cog_kernel(FusedArgs a_) {
       wj[qq] = in ? (float)J : 0.f;
     }
   }
-  // frame and active index of a pair's first item (the pair index is wave-uniform: one scalar
-  // division per pair)
-  auto pair_ref = [&](long pr, int &f0, int &idx0) {
-    const long first = 2 * pr;
-    const long q = first / a.nactive;
-    f0 = (int)q;
-    idx0 = (int)(first - q * a.nactive);
-  };
-  auto next_item = [&](int f0, int idx0, int u, int &f, int &idx) {
-    idx = idx0 + u;
-    f = f0;
-    if (idx >= a.nactive) {
-      idx -= a.nactive;
-      ++f;
-    }
+  // frame and index within the slice of a pair (the pair index is wave-uniform: one scalar
+  // division per wave, then a fixed step)
+  auto pair_ref = [&](long pr, int &f, int &p) {
+    const long q = pr / pp;
+    f = (int)q;
+    p = (int)(pr - q * pp);
   };
   // stage-1 operand of a pair: lane holds E_s[i][j], s = c/16, j = c%16, i = 8h + t
-  auto load_pair = [&](long pr, int f0, int idx0, float (&ph8)[8], float (&pu8)[8]) {
+  auto load_pair = [&](long pr, int f, int p, float (&ph8)[8], float (&pu8)[8]) {
     const int u = c >> 4;
     // both items' sub-aperture indices and window offsets in scalar registers (scalar-cache
     // reads, scalar division), then the lane's item by a select. Loads are unconditional: an
     // item past the end reads item 0 (its results are never used).
-    int fA, iA, fB, iB;
-    next_item(f0, idx0, 0, fA, iA);
-    next_item(f0, idx0, 1, fB, iB);
-    const bool vA = pr < npairs, vB = vA && 2 * pr + 1 < a.nitems;
-    const int sA = vA ? scalar_read(a.active, iA) : 0, sB = vB ? scalar_read(a.active, iB) : 0;
+    const bool vA = pr < npairs, vB = vA && 2 * p + 1 < cnt;
+    const int fA = f, fB = f;
+    const int sA = vA ? scalar_read(a.active, s0 + 2 * p) : 0;
+    const int sB = vB ? scalar_read(a.active, s0 + 2 * p + 1) : 0;
     auto window = [&](int sa, int fr) {
       const int sy = sa / a.nsubx, sx = sa - sy * a.nsubx;
       return (long)fr * a.phs_stride + (long)(sy * kN) * a.npup + sx * kN;
@@ -128,8 +128,8 @@ __global__ __launch_bounds__(256) void dft16_cog_kernel(FusedArgs a_) {
   for (long pr = wave_id; pr < npairs; pr += nwaves) {
     float nph[8], npu[8];
     int nf0 = cf0 + stepF, nidx0 = cidx0 + stepR;
-    if (nidx0 >= a.nactive) {
-      nidx0 -= a.nactive;
+    if (nidx0 >= pp) {
+      nidx0 -= pp;
       ++nf0;
     }
     load_pair(pr + nwaves, nf0, nidx0, nph, npu);   // next pair's loads overlap this pair's work
@@ -158,10 +158,9 @@ __global__ __launch_bounds__(256) void dft16_cog_kernel(FusedArgs a_) {
 
 #pragma unroll
     for (int sub = 0; sub < 2; ++sub) {
-      if (2 * pr + sub >= a.nitems) break;   // wave-uniform
-      int frame, sidx;
-      next_item(pf0, pidx0, sub, frame, sidx);
-      const int s = scalar_read(a.active, sidx);
+      if (2 * pidx0 + sub >= cnt) break;   // wave-uniform
+      const int frame = pf0;
+      const int s = scalar_read(a.active, s0 + 2 * pidx0 + sub);
       // ---- stage 2: B2 = Z_s^T from accumulator registers 8*sub .. 8*sub+7 ----
       float br[8], bi[8];
 #pragma unroll
@@ -286,12 +285,13 @@ bool dft_supported(int n, int F, int nimg) {
 hipError_t launch_dft(const FusedArgs &a, int bf16, hipStream_t st) {
   if (a.nitems <= 0) return hipSuccess;
   if (a.n != kN) return launch_dft32(a, bf16, st);
-  const long npairs = (a.nitems + 1) / 2;
+  const long npairs = (a.nitems + 1) / 2 + kXcds;
   long blocks = (npairs + 3) / 4;
   // persistent grid: every wave builds the DFT-matrix operands once (hundreds of instructions),
   // so launch about the resident wave count (two waves per SIMD) and grid-stride
   const long cap = 256L * 2;
   if (blocks > cap) blocks = cap;
+  blocks = (blocks + kXcds - 1) / kXcds * kXcds;   // every XCD gets the same number of workgroups
   if (bf16) {
     if (a.fast_trig) hipLaunchKernelGGL((dft16_cog_kernel<b8, true>), dim3((unsigned)blocks), dim3(256), 0, st, a);
     else hipLaunchKernelGGL((dft16_cog_kernel<b8, false>), dim3((unsigned)blocks), dim3(256), 0, st, a);

@@ -20,7 +20,7 @@
 namespace dasp {
 
 namespace {
-constexpr int kF64 = 64, kM64 = 16;
+constexpr int kF64 = 64, kM64 = 16, kXcds = 8;
 }
 
 
@@ -58,14 +58,23 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void d
       }
   }
   __syncthreads();
-  const long wave_id = (long)blockIdx.x * (blockDim.x >> 6) + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const long nwaves = (long)gridDim.x * (blockDim.x >> 6);
+  // XCD-aware item order: workgroup b runs on XCD b % 8, and XCD x takes the x-th eighth of the
+  // active sub-apertures for every frame, so that its L2 keeps that slice's pupil. Within an XCD
+  // the slice's items (frame-major) are taken with a grid stride.
+  const int xcd = blockIdx.x % kXcds;
+  const int s0 = (int)((long)a.nactive * xcd / kXcds);
+  const int cnt = (int)((long)a.nactive * (xcd + 1) / kXcds) - s0;
+  const long nx = cnt > 0 ? (a.nitems / a.nactive) * cnt : 0;   // this XCD's items
+  const long wave_id = (long)(blockIdx.x / kXcds) * (blockDim.x >> 6) + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const long nwaves = (long)(gridDim.x / kXcds) * (blockDim.x >> 6);
+  if (wave_id >= nx) return;   // wave-uniform; the workgroup has no barrier after this point
   const int n = a.n;
-  // item it = f * nactive + idx, it advancing by nwaves (one scalar division per wave)
-  auto split_item = [&](long it, int &f, int &idx) {
-    const long q = it / a.nactive;
-    f = (int)q;
-    idx = (int)(it - q * a.nactive);
+  // local item q = f * cnt + r (sub-aperture s0 + r of frame f), q advancing by nwaves: one
+  // scalar division per wave, then a fixed step
+  auto split_item = [&](long q, int &f, int &r) {
+    const long fq = q / cnt;
+    f = (int)fq;
+    r = (int)(q - fq * cnt);
   };
   int stepF, stepR;
   split_item(nwaves, stepF, stepR);
@@ -73,9 +82,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void d
   // Loads are unconditional (no per-element branches): past the last item the wave reads item 0
   // (never used), and for n < 32 the padding rows/columns read a clamped in-window address with
   // the pupil factor forced to 0, so E is exactly 0 there.
-  auto load_item = [&](long it, int f, int idx, float (&ph)[16], float (&pu)[16]) {
-    const bool v = it < a.nitems;   // wave-uniform
-    const int s = v ? scalar_read(a.active, idx) : 0;
+  auto load_item = [&](long q, int f, int r, float (&ph)[16], float (&pu)[16]) {
+    const bool v = q < nx;   // wave-uniform
+    const int s = v ? scalar_read(a.active, s0 + r) : 0;
     const int fr = v ? f : 0;
     const int sy = s / a.nsubx, sx = s - (s / a.nsubx) * a.nsubx;
     const float *p0 = a.phs + (long)fr * a.phs_stride + (long)(sy * n) * a.npup + sx * n;
@@ -105,13 +114,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void d
   split_item(wave_id, cf, cidx);
   float cph[16], cpu[16];
   load_item(wave_id, cf, cidx, cph, cpu);
-  for (long it = wave_id; it < a.nitems; it += nwaves) {
+  for (long q = wave_id; q < nx; q += nwaves) {
     int nf = cf + stepF, nidx = cidx + stepR;
-    if (nidx >= a.nactive) {
-      nidx -= a.nactive;
+    if (nidx >= cnt) {
+      nidx -= cnt;
       ++nf;
     }
-    const int frame = cf, s = scalar_read(a.active, cidx);
+    const int frame = cf, s = scalar_read(a.active, s0 + cidx);
     // ---- E = pup exp(i (phi - t)) and its hi/lo operands ----
     COp<T8> A1[2];
 #pragma unroll
@@ -158,7 +167,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void d
       }
       // the next item's loads, issued once the stage-1 operands are dead (their registers are
       // not held across the whole item) and overlapping the stage-2 products
-      if (kt == 1) load_item(it + nwaves, nf, nidx, cph, cpu);
+      if (kt == 1) load_item(q + nwaves, nf, nidx, cph, cpu);
 #pragma unroll
       for (int lt = 0; lt < 2; ++lt) {
         f16v xr = {}, xi = {};
@@ -303,6 +312,7 @@ hipError_t launch_dft32(const FusedArgs &a, int bf16, hipStream_t st) {
   // measured no faster
   const long cap = 256L * 2;
   if (blocks > cap) blocks = cap;
+  blocks = (blocks + kXcds - 1) / kXcds * kXcds;   // every XCD gets the same number of workgroups
   if (bf16) {
     if (a.fast_trig) launch_dft32_t<b8, true>(a, blocks, st);
     else launch_dft32_t<b8, false>(a, blocks, st);
