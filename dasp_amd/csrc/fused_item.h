@@ -319,26 +319,32 @@ __device__ __forceinline__ void fused_item(const FusedArgs &a, const ItemRef &it
         px[decltype(ib)::value * NIMG + decltype(jb)::value] = acc[decltype(ib)::value][decltype(jb)::value];
       });
     });
+    // the loops' uniform arguments in registers (no argument-segment reload per pixel)
     const long noff = (long)frame * a.noise_stride + pix0;
     const bool pois = valid && a.poisson_on && a.poisson_on[s];
-    const bool rn = a.readnoise != 0.f && a.zread;
+    const float *skyarr = a.skyarr ? a.skyarr + pix0 : nullptr;
+    const float *zpois = a.zpois + noff, *zread = a.zread ? a.zread + noff : nullptr;
+    const float readnoise = a.readnoise, readbg = a.readbg, pxlPower = a.pxlPower;
+    const int threshType = a.threshType;
+    const bool rn = readnoise != 0.f && zread;
+    const bool addbg = threshType >= 0 && threshType <= 2;
     float mx = -INFINITY;
 #pragma unroll 1
     for (int q = 0; q < RB * NIMG; ++q) {
       const int I = det_row(q / NIMG), jb = q % NIMG;
       float v = px[q] * scale;   // scale is 1 where the theoretical total is not positive
       if (owner) {
-        const long p = I * NIMG + jb;
+        const int p = I * NIMG + jb;
         if (skyadd > 0.f) v += skyadd;
-        if (a.skyarr) v += a.skyarr[pix0 + p];
-        if (pois) v = poisson_from_normal(v, a.zpois[noff + p]);
-        if (a.threshType >= 0 && a.threshType <= 2) v += rn ? fmaf(a.readnoise, a.zread[noff + p], a.readbg) : a.readbg;
+        if (skyarr) v += skyarr[p];
+        if (pois) v = poisson_from_normal(v, zpois[p]);
+        if (addbg) v += rn ? fmaf(readnoise, zread[p], readbg) : readbg;
         mx = fmaxf(mx, v);
       }
       px[q] = v;
     }
     float thr = a.floor;
-    if (a.threshType == 2) {
+    if (threshType == 2) {
       mx = group_max<NP>(mx);
       if (a.floor > 0.f && a.floor <= 1.f) thr = a.floor * fmaxf(mx, 0.f);
     }
@@ -348,10 +354,10 @@ __device__ __forceinline__ void fused_item(const FusedArgs &a, const ItemRef &it
     for (int q = 0; q < RB * NIMG; ++q) {
       const int I = det_row(q / NIMG), jb = q % NIMG;
       float v = px[q];
-      if (a.threshType == 1) v = v < thr ? 0.f : v;
-      else if (a.threshType == 0 || a.threshType == 2) v = v < thr ? 0.f : v - thr;
-      if (a.pxlPower == 2.f) v = v * v;
-      else if (a.pxlPower != 1.f && a.pxlPower != 0.f) v = powf(v, a.pxlPower);
+      if (threshType == 1) v = v < thr ? 0.f : v;
+      else if (threshType == 0 || threshType == 2) v = v < thr ? 0.f : v - thr;
+      if (pxlPower == 2.f) v = v * v;
+      else if (pxlPower != 1.f && pxlPower != 0.f) v = powf(v, pxlPower);
       if (owner) {
         if (bout) bout[I * NIMG + jb] = v;
         if (I >= st0 && I < en0 && jb >= st0 && jb < en0) {

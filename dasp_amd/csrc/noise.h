@@ -1,4 +1,4 @@
-// noise.h - photon noise from a pre-drawn standard normal, shared by host and device code.
+// noise.h - photon noise from a pre-drawn standard normal (device code of every image path).
 //
 // The reference draws Poisson variates inside the module (rk_poisson, cmod/src/centmodule.c:
 // 760-782: multiplication method below lambda=10, PTRS rejection above) from per-thread PCG32
@@ -16,18 +16,30 @@
 
 namespace dasp {
 
-__host__ __device__ inline float poisson_from_normal(float lam, float z) {
+// 1/(k+1) for the inverse-CDF recurrence, folded at compile time (correctly rounded)
+static __constant__ const float kPoisInv[40] = {
+    1.0f / 1,  1.0f / 2,  1.0f / 3,  1.0f / 4,  1.0f / 5,  1.0f / 6,  1.0f / 7,  1.0f / 8,
+    1.0f / 9,  1.0f / 10, 1.0f / 11, 1.0f / 12, 1.0f / 13, 1.0f / 14, 1.0f / 15, 1.0f / 16,
+    1.0f / 17, 1.0f / 18, 1.0f / 19, 1.0f / 20, 1.0f / 21, 1.0f / 22, 1.0f / 23, 1.0f / 24,
+    1.0f / 25, 1.0f / 26, 1.0f / 27, 1.0f / 28, 1.0f / 29, 1.0f / 30, 1.0f / 31, 1.0f / 32,
+    1.0f / 33, 1.0f / 34, 1.0f / 35, 1.0f / 36, 1.0f / 37, 1.0f / 38, 1.0f / 39, 1.0f / 40};
+
+__device__ inline float poisson_from_normal(float lam, float z) {
   if (!(lam > 0.f)) return 0.f;
   if (lam < 10.f) {
     const float u = 0.5f * erfcf(-z * 0.70710678118654752f);
     float p = expf(-lam);
     float F = p;
+    // Every lane still searching has taken exactly i steps at step i, so the step count is
+    // wave-uniform and 1/(k+1) comes from the table through the scalar cache; the loop runs
+    // until the wave's last lane stops, so its body is kept short.
     int k = 0;
-    while (u > F && k < 40) {
-      const float pn = p * lam / (float)(k + 1);
+    for (int i = 0; i < 40; ++i) {
+      if (!(u > F)) break;
+      const float pn = p * (lam * kPoisInv[i]);
       const float Fn = F + pn;
       if (!(Fn > F)) break;
-      ++k;
+      k = i + 1;
       p = pn;
       F = Fn;
     }
