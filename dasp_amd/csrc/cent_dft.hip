@@ -342,8 +342,10 @@ __global__ __launch_bounds__(256) void dft16_pipe_kernel(FusedArgs a_) {
   auto load_pair = [&](long pr, int f, int p, float (&ph8)[8], float (&pu8)[8]) {
     const int u = c >> 4;
     const bool vA = pr < npairs, vB = vA && 2 * p + 1 < cnt;
-    const int sA = vA ? scalar_read(a.active, s0 + 2 * p) : 0;
-    const int sB = vB ? scalar_read(a.active, s0 + 2 * p + 1) : 0;
+    // unconditional in-slice reads (p < pp), selected afterwards: no branch in the loop body
+    const int rA = scalar_read(a.active, s0 + 2 * p);
+    const int rB = scalar_read(a.active, s0 + min(2 * p + 1, cnt - 1));
+    const int sA = vA ? rA : 0, sB = vB ? rB : 0;
     auto window = [&](int sa, int fr) {
       const int sy = sa / a.nsubx, sx = sa - sy * a.nsubx;
       return (long)fr * a.phs_stride + (long)(sy * kN) * a.npup + sx * kN;
@@ -396,7 +398,12 @@ __global__ __launch_bounds__(256) void dft16_pipe_kernel(FusedArgs a_) {
   advance(nf, np);
   load_pair(wave_id + nwaves, nf, np, ph8, pu8);
   for (long q = wave_id; q < npairs; q += nwaves) {
-    // ---- stage 2 of pair q (both items) ----
+    // Three scheduling regions (nothing crosses a sched_barrier), each pairing matrix-core work
+    // with independent vector work: (1) stage 2 of pair q with the phase factor of the pair
+    // after it, (2) stage 1 of that pair with the detector of item 0 of pair q, (3) the detector
+    // of item 1, the wave sums and the loads two pairs ahead.
+    const int f = cf, p = cp;
+    // ---- region 1 ----
     COp<T8> B2[2];
 #pragma unroll
     for (int sub = 0; sub < 2; ++sub) {
@@ -411,27 +418,18 @@ __global__ __launch_bounds__(256) void dft16_pipe_kernel(FusedArgs a_) {
     f16v xr0 = {}, xi0 = {}, xr1 = {}, xi1 = {};
     cgemm_cv(A2, B2[0], xr0, xi0);
     cgemm_cv(A2, B2[1], xr1, xi1);
-    // ---- stage 1 of pair q + nwaves (its loads were issued one iteration ago) ----
-    {
-      COp<T8> A1;
-      phase_factor(ph8, pu8, A1);
-      zr = f16v{};
-      zi = f16v{};
-      cgemm_vc(A1, B1, zr, zi);
-    }
-    const int f = cf, p = cp;
-    cf = nf;
-    cp = np;
-    advance(nf, np);
-    load_pair(q + 2 * nwaves, nf, np, ph8, pu8);   // two pairs ahead
-    // ---- detector of pair q: |X|^2 binned 2x2, normalisation, threshold, CoG sums ----
+    COp<T8> A1n;
+    phase_factor(ph8, pu8, A1n);   // pair q + nwaves (loaded one iteration ago)
+    __builtin_amdgcn_sched_barrier(0);
+    // ---- region 2 ----
+    zr = f16v{};
+    zi = f16v{};
+    cgemm_vc(A1n, B1, zr, zi);
+    // detector of pair q: |X|^2 binned 2x2, normalisation, threshold, CoG sums
     float sums[2][3];
     float img[2][4];
     int sidx[2];
-#pragma unroll
-    for (int sub = 0; sub < 2; ++sub) {
-      const f16v &xr = sub ? xr1 : xr0;
-      const f16v &xi = sub ? xi1 : xi0;
+    auto detector = [&](int sub, const f16v &xr, const f16v &xi) {
       float pix[8];
 #pragma unroll
       for (int r8 = 0; r8 < 8; ++r8) {
@@ -458,11 +456,19 @@ __global__ __launch_bounds__(256) void dft16_pipe_kernel(FusedArgs a_) {
       sums[sub][0] = sw;
       sums[sub][1] = sxw;
       sums[sub][2] = fI * sw;   // the lane's pixels share one detector row
-    }
+    };
+    detector(0, xr0, xi0);
+    __builtin_amdgcn_sched_barrier(0);
+    // ---- region 3 ----
+    detector(1, xr1, xi1);
 #pragma unroll
     for (int sub = 0; sub < 2; ++sub)
 #pragma unroll
       for (int k = 0; k < 3; ++k) sums[sub][k] = wave_total(sums[sub][k]);
+    cf = nf;
+    cp = np;
+    advance(nf, np);
+    load_pair(q + 2 * nwaves, nf, np, ph8, pu8);   // two pairs ahead
     // ---- stores ----
     const int nvalid = 2 * p + 1 < cnt ? 2 : 1;
     if (a.bimg) {
