@@ -122,7 +122,7 @@ struct dasp_cent {
   int inputDims = 0, inputNpxlx = 0;
   long phsElems = 0;
   // device state
-  DevBuf d_phs, d_pup, d_tilt, d_active, d_inactive, d_scale, d_totsig, d_sigv, d_skyadd, d_skyarr;
+  DevBuf d_phs, d_pup, d_tilt, d_active, d_winoff, d_inactive, d_scale, d_totsig, d_sigv, d_skyadd, d_skyarr;
   DevBuf d_poisson, d_binA, d_otf, d_corrK, d_corrPat, d_centWeight, d_refCents, d_calData;
   DevBuf d_calBounds, d_calSteps, d_calCoeff, d_ubArr, d_input, d_subapLoc, d_cents, d_bimg;
   DevBuf d_corrimg, d_hll, d_zpois, d_zread, d_bscratch, d_cscratch;
@@ -482,6 +482,7 @@ static void fused_args(dasp_cent *c, const float *d_phs, long phs_stride, int nf
   fa.tiltstep = (float)(M_PI / c->F * (c->F + 1.));
   fa.fast_trig = c->fastTrig;
   fa.active = c->d_active.as<int>();
+  fa.winoff = c->d_winoff.as<int>();
   fa.nactive = (int)c->active.size();
   fa.nitems = nitems;
   fa.nsubx = c->nsubx;
@@ -1100,10 +1101,16 @@ int dasp_cent_initialise(const dasp_cent_params *pp, dasp_cent_t **out) {
   for (int i = 0; i < c->n; ++i)
     for (int j = 0; j < c->n; ++j) c->tilt[i * c->n + j] = tmp * (float)(i + j + 1 - c->n);
   for (int s = 0; s < c->nsubaps; ++s) (c->subflag[s] == 1 ? c->active : c->inactive).push_back(s);
+  std::vector<int> winoff(c->active.size());   // phase-window offsets, read by the DFT-GEMM kernels
+  for (size_t k = 0; k < c->active.size(); ++k) {
+    const int s = c->active[k], sy = s / c->nsubx, sx = s % c->nsubx;
+    winoff[k] = sy * c->n * c->npup + sx * c->n;
+  }
   compute_totsig(c);
   const int M2 = c->nimg * c->nimg;
   if (upload(c->d_pup, c->pupfn.data(), c->pupfn.size() * 4) || upload(c->d_tilt, c->tilt.data(), c->tilt.size() * 4) ||
       upload(c->d_active, c->active.data(), c->active.size() * 4) ||
+      upload(c->d_winoff, winoff.data(), winoff.size() * 4) ||
       upload(c->d_inactive, c->inactive.data(), c->inactive.size() * 4) ||
       c->d_cents.ensure((size_t)c->nsubaps * 2 * 4) || upload(c->d_bimg, c->h_bimg, (size_t)c->nsubaps * M2 * 4))
     return bail(fail(DASP_CENT_EHIP, "device allocation/upload failed"));
@@ -1120,7 +1127,7 @@ int dasp_cent_free(dasp_cent_t *c) {
   if (!c) return 0;
   for (hipEvent_t e : c->evs) (void)hipEventDestroy(e);
   c->evs.clear();
-  DevBuf *bufs[] = {&c->d_phs, &c->d_pup, &c->d_tilt, &c->d_active, &c->d_inactive, &c->d_scale, &c->d_totsig,
+  DevBuf *bufs[] = {&c->d_phs, &c->d_pup, &c->d_tilt, &c->d_active, &c->d_winoff, &c->d_inactive, &c->d_scale, &c->d_totsig,
                     &c->d_sigv, &c->d_skyadd, &c->d_skyarr, &c->d_poisson, &c->d_binA, &c->d_otf, &c->d_corrK,
                     &c->d_corrPat, &c->d_centWeight, &c->d_refCents, &c->d_calData, &c->d_calBounds,
                     &c->d_calSteps, &c->d_calCoeff, &c->d_ubArr, &c->d_input, &c->d_subapLoc, &c->d_cents,

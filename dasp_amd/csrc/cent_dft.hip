@@ -342,15 +342,13 @@ __global__ __launch_bounds__(256) void dft16_pipe_kernel(FusedArgs a_) {
   auto load_pair = [&](long pr, int f, int p, float (&ph8)[8], float (&pu8)[8]) {
     const int u = c >> 4;
     const bool vA = pr < npairs, vB = vA && 2 * p + 1 < cnt;
-    // unconditional in-slice reads (p < pp), selected afterwards: no branch in the loop body
-    const int rA = scalar_read(a.active, s0 + 2 * p);
-    const int rB = scalar_read(a.active, s0 + min(2 * p + 1, cnt - 1));
+    // unconditional in-slice reads (p < pp), selected afterwards: no branch in the loop body;
+    // window offsets from the per-sub-aperture table (no division)
+    const int iA = s0 + 2 * p, iB = s0 + min(2 * p + 1, cnt - 1);
+    const int rA = scalar_read(a.active, iA), rB = scalar_read(a.active, iB);
+    const int oA = scalar_read(a.winoff, iA), oB = scalar_read(a.winoff, iB);
     const int sA = vA ? rA : 0, sB = vB ? rB : 0;
-    auto window = [&](int sa, int fr) {
-      const int sy = sa / a.nsubx, sx = sa - sy * a.nsubx;
-      return (long)fr * a.phs_stride + (long)(sy * kN) * a.npup + sx * kN;
-    };
-    const long wA = window(sA, vA ? f : 0), wB = window(sB, vB ? f : 0);
+    const long wA = vA ? (long)f * a.phs_stride + oA : 0, wB = vB ? (long)f * a.phs_stride + oB : 0;
     const float *ph = a.phs + (u ? wB : wA) + jj;
     const float *pu = a.pup + (long)(u ? sB : sA) * (kN * kN) + jj;
 #pragma unroll

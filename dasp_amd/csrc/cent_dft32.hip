@@ -85,9 +85,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void d
   auto load_item = [&](long q, int f, int r, float (&ph)[16], float (&pu)[16]) {
     const bool v = q < nx;   // wave-uniform
     const int s = v ? scalar_read(a.active, s0 + r) : 0;
-    const int fr = v ? f : 0;
-    const int sy = s / a.nsubx, sx = s - (s / a.nsubx) * a.nsubx;
-    const float *p0 = a.phs + (long)fr * a.phs_stride + (long)(sy * n) * a.npup + sx * n;
+    const float *p0 = a.phs + (v ? (long)f * a.phs_stride + scalar_read(a.winoff, s0 + r) : 0);
     const float *u0 = a.pup + (long)s * n * n;
     if constexpr (FULL) {
       // wave-uniform row bases (scalar registers) plus one 32-bit lane offset

@@ -128,6 +128,7 @@ struct FusedArgs {
   int fast_trig;         // 1: Cody-Waite reduction + hardware sin/cos instead of sincosf
   int slow_detector;     // any of noise, sky array, relative floor, power, weights
   const int *active;
+  const int *winoff;     // per active index: (sy*n)*npup + sx*n, the phase window's offset in a frame
   int nactive;
   long nitems;
   int nsubx, n, npup, nimg, ncen;
