@@ -88,7 +88,7 @@ void batch_kernel(BatchArgs b, int chunk) {
       const bool more = gn < b.total_groups;
       const int dn = more && gn >= end ? desc_of(b, gn) : d;
       PhaseCol<NP> nxt;
-      ItemRef itn{0, 0, false};
+      ItemRef itn{0, 0, 0, false};
       if (more) {
         itn = item_flat(desc_args(b, dn), (gn - b.desc[dn].group0) * SPW, sl);
         load_item<NP>(desc_args(b, dn), itn, j, nxt);

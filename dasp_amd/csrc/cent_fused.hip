@@ -48,14 +48,15 @@ void fused_cog_kernel(FusedArgs a_) {
     const long f = qq / nsgx;
     return item_at(a, qq < ngx ? (int)f : -1, (int)((sg0 + qq - f * nsgx) * SPW) + sl);
   };
+  auto frame_of = [&](long qq) { return qq < ngx ? (int)(qq / nsgx) : 0; };   // wave-uniform
   cf *st = &stage[w][sl * SUB];
   PhaseCol<NP> cur;
   ItemRef it = item_of(q);
-  load_item<NP>(a, it, j, cur);
+  load_item<NP, true>(a, it, j, cur, frame_of(q));
   for (; q < ngx; q += stride) {
     PhaseCol<NP> nxt;
     const ItemRef itn = item_of(q + stride);
-    load_item<NP>(a, itn, j, nxt);   // past the end: zeros
+    load_item<NP, true>(a, itn, j, nxt, frame_of(q + stride));   // past the end: item 0, unused
     fused_item<NP, F, B, FAST, SPOT>(a, it, j, w, sl, cur, st);
     cur = nxt;
     it = itn;

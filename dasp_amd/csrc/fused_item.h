@@ -61,6 +61,7 @@ struct PhaseCol {
 struct ItemRef {
   int frame;
   int s;
+  int off;   // the phase window's offset in its frame, (sy*n)*npup + sx*n
   bool valid;
 };
 
@@ -71,11 +72,14 @@ __device__ __forceinline__ long wave_uniform(unsigned long long v) {
 }
 
 // Item idx (index into the active list) of frame `frame`; idx >= nactive or frame < 0 is padding.
+// Padding reads item 0 (unconditional loads, no branch) and is never written.
 __device__ __forceinline__ ItemRef item_at(const FusedArgs &a, int frame, int idx) {
   ItemRef r;
   r.valid = frame >= 0 && idx < a.nactive;
+  const int ci = r.valid ? idx : 0;
   r.frame = r.valid ? frame : 0;
-  r.s = r.valid ? a.active[idx] : 0;
+  r.s = a.active[ci];
+  r.off = a.winoff[ci];
   return r;
 }
 
@@ -94,21 +98,42 @@ __device__ __forceinline__ ItemRef item_flat(const FusedArgs &a, long first, int
   return item_at(a, in ? f : -1, idx);
 }
 
-// Load phase column j (rows i < n) and the pupil of one work item; zeros where out of range.
-template <int NP>
-__device__ __forceinline__ void load_item(const FusedArgs &a, const ItemRef &it, int j, PhaseCol<NP> &c) {
-  const bool valid = it.valid;
-  const int frame = it.frame, s = it.s;
+// Load phase column j (rows i < n) and the pupil of one work item. The loads are unconditional
+// (no per-element branches): with n < NP the padding rows/columns read a clamped in-window
+// address and the pupil factor is zeroed by a product, so E is exactly 0 there.
+// With UF the caller's frame fw is the same for the whole wave (the fused kernel's order): the
+// row bases are then scalar and each load takes one 32-bit lane offset (no 64-bit address
+// arithmetic per element).
+template <int NP, bool UF = false>
+__device__ __forceinline__ void load_item(const FusedArgs &a, const ItemRef &it, int j, PhaseCol<NP> &c, int fw = 0) {
   const int n = a.n;
-  const int sy = s / a.nsubx, sx = s - (s / a.nsubx) * a.nsubx;
-  const float *ph = a.phs + (long)frame * a.phs_stride + (long)(sy * n) * a.npup + sx * n;
-  const float *pu = a.pup + (long)s * n * n;
-  static_for<0, NP>([&](auto ii) {
-    constexpr int i = decltype(ii)::value;
-    const bool in = valid && i < n && j < n;
-    c.ph[i] = in ? ph[(long)i * a.npup + j] : 0.f;
-    c.pu[i] = in ? pu[i * n + j] : 0.f;
-  });
+  const float *ph = a.phs + (long)it.frame * a.phs_stride + it.off;
+  const float *pu = a.pup + (long)it.s * n * n;
+  if (n == NP) {
+    if constexpr (UF) {
+      const float *base = a.phs + (long)fw * a.phs_stride;
+      const int po = it.off + j, uo = it.s * (NP * NP) + j;
+      static_for<0, NP>([&](auto ii) {
+        constexpr int i = decltype(ii)::value;
+        c.ph[i] = (base + (long)i * a.npup)[po];
+        c.pu[i] = (a.pup + i * NP)[uo];
+      });
+    } else {
+      static_for<0, NP>([&](auto ii) {
+        constexpr int i = decltype(ii)::value;
+        c.ph[i] = ph[(long)i * a.npup + j];
+        c.pu[i] = pu[i * NP + j];
+      });
+    }
+  } else {
+    const int jc = j < n ? j : n - 1;
+    static_for<0, NP>([&](auto ii) {
+      constexpr int i = decltype(ii)::value;
+      const int ic = i < n ? i : n - 1;
+      c.ph[i] = ph[(long)ic * a.npup + jc];
+      c.pu[i] = pu[ic * n + jc] * ((i < n && j < n) ? 1.f : 0.f);
+    });
+  }
 }
 
 // LDS complex values per subaperture: the pass-1 transpose (NP rows of NP + 1), or with a spot
