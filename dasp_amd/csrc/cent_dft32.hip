@@ -89,12 +89,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void d
     const float *u0 = a.pup + (long)s * n * n;
     if constexpr (FULL) {
       // wave-uniform row bases (scalar registers) plus one 32-bit lane offset
-      const int po = 8 * h * a.npup + c, uo = 8 * h * 32 + c;
+      // 32-bit unsigned byte offsets (zero-extended), so that the loads take the scalar-base
+      // plus vector-offset form
+      const unsigned po = (unsigned)(8 * h * a.npup + c) * 4u, uo = (unsigned)(8 * h * 32 + c) * 4u;
 #pragma unroll
       for (int q = 0; q < 16; ++q) {
         const int r = 16 * (q >> 3) + (q & 7);
-        ph[q] = (p0 + (long)r * a.npup)[po];
-        pu[q] = (u0 + r * 32)[uo];
+        ph[q] = *(const float *)((const char *)(p0 + (long)r * a.npup) + po);
+        pu[q] = *(const float *)((const char *)(u0 + r * 32) + uo);
       }
     } else {
       // the zero factor is a product, not a select, so that the load is not made conditional

@@ -44,22 +44,33 @@ void fused_cog_kernel(FusedArgs a_) {
   const long stride = (long)(gridDim.x / kXcds) * S::WAVES;
   long q = (long)(blockIdx.x / kXcds) * S::WAVES + w;
   if (q >= ngx) return;           // wave-uniform
-  auto item_of = [&](long qq) -> ItemRef {   // qq is wave-uniform: scalar division
-    const long f = qq / nsgx;
-    return item_at(a, qq < ngx ? (int)f : -1, (int)((sg0 + qq - f * nsgx) * SPW) + sl);
+  // group q = f * nsgx + g (group sg0 + g of frame f); q advances by a fixed stride, so frame
+  // and group advance by fixed steps: one scalar division per wave
+  const int nsgi = (int)nsgx;
+  int fcur = (int)(q / nsgx), gcur = (int)(q - (long)fcur * nsgx);
+  const int stepF = (int)(stride / nsgx), stepG = (int)(stride - (long)stepF * nsgx);
+  auto item_of = [&](long qq, int f, int g) -> ItemRef {   // wave-uniform arguments
+    return item_at(a, qq < ngx ? f : -1, (int)((sg0 + g) * SPW) + sl);
   };
-  auto frame_of = [&](long qq) { return qq < ngx ? (int)(qq / nsgx) : 0; };   // wave-uniform
   cf *st = &stage[w][sl * SUB];
   PhaseCol<NP> cur;
-  ItemRef it = item_of(q);
-  load_item<NP, true>(a, it, j, cur, frame_of(q));
+  ItemRef it = item_of(q, fcur, gcur);
+  load_item<NP, true>(a, it, j, cur, fcur);
   for (; q < ngx; q += stride) {
+    int nf = fcur + stepF, ng = gcur + stepG;
+    if (ng >= nsgi) {
+      ng -= nsgi;
+      ++nf;
+    }
     PhaseCol<NP> nxt;
-    const ItemRef itn = item_of(q + stride);
-    load_item<NP, true>(a, itn, j, nxt, frame_of(q + stride));   // past the end: item 0, unused
+    const bool more = q + stride < ngx;
+    const ItemRef itn = item_of(q + stride, nf, ng);
+    load_item<NP, true>(a, itn, j, nxt, more ? nf : 0);   // past the end: item 0, unused
     fused_item<NP, F, B, FAST, SPOT>(a, it, j, w, sl, cur, st);
     cur = nxt;
     it = itn;
+    fcur = nf;
+    gcur = ng;
   }
 }
 

@@ -112,11 +112,13 @@ __device__ __forceinline__ void load_item(const FusedArgs &a, const ItemRef &it,
   if (n == NP) {
     if constexpr (UF) {
       const float *base = a.phs + (long)fw * a.phs_stride;
-      const int po = it.off + j, uo = it.s * (NP * NP) + j;
+      // 32-bit unsigned byte offsets (zero-extended), so that the loads take the scalar-base
+      // plus vector-offset form
+      const unsigned po = (unsigned)(it.off + j) * 4u, uo = (unsigned)(it.s * (NP * NP) + j) * 4u;
       static_for<0, NP>([&](auto ii) {
         constexpr int i = decltype(ii)::value;
-        c.ph[i] = (base + (long)i * a.npup)[po];
-        c.pu[i] = (a.pup + i * NP)[uo];
+        c.ph[i] = *(const float *)((const char *)(base + (long)i * a.npup) + po);
+        c.pu[i] = *(const float *)((const char *)(a.pup + i * NP) + uo);
       });
     } else {
       static_for<0, NP>([&](auto ii) {
