@@ -281,6 +281,61 @@ def test_dft_mfma_form_error_vs_fp64(form):
     c.free()
 
 
+DFT16_CASES = {
+    # name: Case keyword arguments (n=16, nfft=32, nimg=16); the pipelined kernel runs the
+    # inline detector, the general one noise and relative floors
+    "floor_type0": dict(noiseFloor=0.004, threshType=0),
+    "floor_type1": dict(noiseFloor=0.004, threshType=1),
+    "ncen12_sky": dict(ncen=12, skybrightness=0.001),
+    "noise_relfloor": dict(sig=1000.0, readnoise=0.5, addPoisson=1, noiseFloor=0.1, threshType=2, seed=4),
+}
+
+
+@pytest.mark.parametrize("form", ["fp16", "bf16"])
+@pytest.mark.parametrize("name", sorted(DFT16_CASES))
+def test_dft16_options_vs_fp64(name, form):
+    """The n=16/nfft=32 DFT-GEMM form with detector options, 11x11 WFS (an odd active count, so
+    that XCD slices end in half-empty pairs), images and slopes against the FP64 oracle."""
+    from dasp_amd.cmod import cent
+    say("case DFT16 %s %s" % (name, form))
+    wgeo = W.WFS("dft16", 11, 16, 32, 16, 8.0, 0.2, 0.15, 25.0)
+    kw = dict(DFT16_CASES[name])
+    c = Case(11, 16, 32, 16, obs=0.2, phs=wgeo.phase(),
+             form={"fp16": cent.FORM_DFT_FP16, "bf16": cent.FORM_DFT_BF16}[form], **kw)
+    oc, ob, _ = c.run_both()
+    act = _active(c)
+    say("active sub-apertures: %d" % act.sum())
+    b = c.bimg.reshape(-1, 16, 16)[act]
+    tol = 5e-5 if form == "fp16" else 5e-4
+    # a pixel within the form's error of a threshold may fall on the other side of it
+    diffpix = np.abs(b - ob[act]) > tol * np.abs(ob[act]).max()
+    say("pixels beyond %.0e of the maximum: %d of %d" % (tol, diffpix.sum(), diffpix.size))
+    assert diffpix.sum() <= max(2, 2e-3 * diffpix.size)
+    clean = ~diffpix.reshape(diffpix.shape[0], -1).any(axis=1)
+    d = np.abs(c.cents.reshape(-1, 2)[act] - oc[act])[clean]
+    assert np.sqrt((d ** 2).mean()) < 1e-3
+    if form == "fp16" and "noise" not in name:
+        assert d.max() < 1e-4
+    assert np.all(c.cents.reshape(-1, 2)[~act] == 0)
+    c.free()
+
+
+def test_dft16_refcents():
+    """Reference slopes subtracted in the pipelined DFT kernel (REFCENTS update)."""
+    from dasp_amd.cmod import cent
+    say("case DFT16 refcents")
+    wgeo = W.WFS("dft16", 11, 16, 32, 16, 8.0, 0.2, 0.15, 25.0)
+    c = Case(11, 16, 32, 16, obs=0.2, form=cent.FORM_DFT_FP16)
+    c.run_both()
+    ref = c.cents.copy()
+    c.phs[:] = wgeo.phase()
+    c.cc.update(18, ref)   # REFCENTS
+    c.params.refCents = ref
+    oc, _, _ = c.run_both()
+    assert report("dft16-refcents", c.cents, oc, _active(c)).max() < 1e-4
+    c.free()
+
+
 DFT32_CASES = {
     # name: (nsubx, n, correlation + parabolic fit, noise)
     "n32_cog": (16, 32, False, False),
