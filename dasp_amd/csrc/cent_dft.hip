@@ -17,6 +17,8 @@
 // The accumulator of stage 2 holds X[k][l] with lane = k and l = 8g + 4h + u (g, u < 4): |X|^2
 // is binned in-lane along l and across lane pairs along k, then the detector steps and the CoG
 // follow as in the fused FP32 kernel.
+// Two kernels: dft16_pipe_kernel (software pipelined, inline detector only) and dft16_cog_kernel
+// (every detector option, including noise and relative floors).
 #include "dft_ops.h"
 #include "noise.h"
 #include "wave_ops.h"
