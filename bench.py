@@ -4,7 +4,8 @@ Default workload: the 80x80 WFS at nfft=32 (D=39 m, 0.28 obstruction, 1280x1280 
 nimg=16, noise off), 64 frames of synthetic von Karman phase per step, resident in HBM before the
 timed region. One step = one device-resident engine call over the 64 frames. Prints one JSON line.
 
-python bench.py [--gpus 1] [--steps K] [--warmup W] [--form fft32|bf16|fp16] [--workload c3|c2|c5|c1]
+python bench.py [--gpus 1] [--steps K] [--warmup W] [--form fft32|bf16|fp16]
+                [--workload wfs80x80_nfft32|wfs40x40_noise|wfs80x80_nyquist_corr|multi_wfs|scao8x8]
 """
 import argparse
 import json
@@ -114,7 +115,9 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--form", default="fft32", choices=["fft32", "bf16", "fp16"])
-    ap.add_argument("--workload", default="c3", choices=["c1", "c2", "c3", "c5", "c4"])
+    workloads = {"scao8x8": "SCAO8", "wfs40x40_noise": "NOISE40", "wfs80x80_nfft32": "WFS80",
+                 "wfs80x80_nyquist_corr": "CORR80"}   # name -> dasp_amd.workloads attribute
+    ap.add_argument("--workload", default="wfs80x80_nfft32", choices=sorted(workloads) + ["multi_wfs"])
     ap.add_argument("--tune-cache", default=None,
                     help="JSON file of autotuned multi-WFS launch configurations (read if present, else written)")
     ap.add_argument("--batch", action="store_true",
@@ -140,9 +143,9 @@ def main():
 
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
-    if args.workload == "c4":
+    if args.workload == "multi_wfs":
         return bench_multi(args, dev)
-    w = {"c1": W.C1, "c2": W.C2, "c3": W.C3, "c5": W.C5}[args.workload]
+    w = getattr(W, workloads[args.workload])
     frames = args.frames or w.nframes
     S = w.nsubx * w.nsubx
     pupil = w.pupil()
@@ -302,7 +305,7 @@ def bench_multi(args, dev):
     from dasp_amd.util import centcmod
     from dasp_amd.wfsbatch import WFSBatch
     frames = args.frames or 16
-    geo = [(W.C4_LGS, k) for k in range(6)] + [(W.C4_NGS, 6 + k) for k in range(3)] + [(W.C4_TRUTH, 9)]
+    geo = [(W.LGS80, k) for k in range(6)] + [(W.NGS8, 6 + k) for k in range(3)] + [(W.TRUTH40, 9)]
     ccs, hosts, spot_cache = [], [], {}
     for w, k in geo:
         pupil = w.pupil()

@@ -66,16 +66,17 @@ class WFS:
                                telDiam=self.D, pup=self.pupil().fn)
 
 
-# The five configurations of the benchmark definition.
-C1 = WFS("scao8x8", 8, 8, 16, 8, 4.2, 0.25, 0.15, 30.0)
-C2 = WFS("wfs40x40_noise", 40, 16, 32, 16, 8.0, 0.14, 0.12, 25.0, sig=1000.0, readnoise=0.5,
+# The benchmark workloads: SCAO 8x8, 40x40 with noise, 80x80 nfft32, the multi-WFS set
+# (LGS 80x80, NGS 8x8, truth 40x40) and the 80x80 Nyquist WFS with correlation.
+SCAO8 = WFS("scao8x8", 8, 8, 16, 8, 4.2, 0.25, 0.15, 30.0)
+NOISE40 = WFS("wfs40x40_noise", 40, 16, 32, 16, 8.0, 0.14, 0.12, 25.0, sig=1000.0, readnoise=0.5,
          addPoisson=1, noiseFloor=0.1, threshType=2, nframes=64)
-C3 = WFS("wfs80x80_nfft32", 80, 16, 32, 16, 39.0, 0.28, 0.157, 25.0, nframes=64)
-C4_LGS = WFS("lgs80x80", 80, 16, 32, 16, 39.0, 0.28, 0.157, 25.0, sig=1000.0, readnoise=0.5,
+WFS80 = WFS("wfs80x80_nfft32", 80, 16, 32, 16, 39.0, 0.28, 0.157, 25.0, nframes=64)
+LGS80 = WFS("lgs80x80", 80, 16, 32, 16, 39.0, 0.28, 0.157, 25.0, sig=1000.0, readnoise=0.5,
              addPoisson=1, spot=True)
-C4_NGS = WFS("ngs8x8", 8, 32, 64, 8, 39.0, 0.28, 0.157, 25.0, sig=1000.0, readnoise=0.5, addPoisson=1)
-C4_TRUTH = WFS("truth40x40", 40, 16, 32, 16, 39.0, 0.28, 0.157, 25.0, sig=1000.0, readnoise=0.5, addPoisson=1)
-C5 = WFS("wfs80x80_nyquist_corr", 80, 32, 64, 16, 39.0, 0.28, 0.157, 25.0, correlation=1, parabolicFit=5,
+NGS8 = WFS("ngs8x8", 8, 32, 64, 8, 39.0, 0.28, 0.157, 25.0, sig=1000.0, readnoise=0.5, addPoisson=1)
+TRUTH40 = WFS("truth40x40", 40, 16, 32, 16, 39.0, 0.28, 0.157, 25.0, sig=1000.0, readnoise=0.5, addPoisson=1)
+CORR80 = WFS("wfs80x80_nyquist_corr", 80, 32, 64, 16, 39.0, 0.28, 0.157, 25.0, correlation=1, parabolicFit=5,
          nframes=256)
 
 

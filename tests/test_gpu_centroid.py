@@ -14,7 +14,7 @@ pytestmark = pytest.mark.gpu
 
 
 def _make(device, **kw):
-    w = W.C1
+    w = W.SCAO8
     args = dict(nimg=8, sig=500., readnoise=0.5, addPoisson=1, noiseFloor=0.05, seed=3, device=device)
     args.update(kw)
     return new_centroid(w.nsubx, w.pupil(), w.n, w.nfft, **args)
@@ -31,7 +31,7 @@ def test_generate_next_host_and_device_routes_agree():
     hc, dc = _make(None, **kw), _make("cuda:0", **kw)
     mh, md = wfscent(hc, {"calcCovariance": 1}), wfscent(dc, {"calcCovariance": 1})
     for k in range(6):
-        phs = W.C1.phase(k)
+        phs = W.SCAO8.phase(k)
         mh.generateNext(phs.copy())
         md.generateNext(torch.from_numpy(phs).cuda())
         torch.cuda.synchronize()

@@ -27,7 +27,7 @@ PATHS = {"fused": 0, "general": 3, "fused_fasttrig": 10}
 
 
 @pytest.mark.parametrize("path", sorted(PATHS))
-@pytest.mark.parametrize("wfs", [W.C1, W.C3], ids=["c1_8x8", "c3_80x80"])
+@pytest.mark.parametrize("wfs", [W.SCAO8, W.WFS80], ids=["scao8x8", "wfs80x80"])
 def test_noiseless_cog_vs_oracle(wfs, path):
     say("case noiseless CoG %s %s" % (wfs.name, path))
     c = Case(wfs.nsubx, wfs.n, wfs.nfft, wfs.nimg, obs=wfs.obs, phs=wfs.phase(), form=PATHS[path])
@@ -44,7 +44,7 @@ def test_noiseless_cog_vs_oracle(wfs, path):
 
 def test_flat_phase_reference_and_refcents():
     say("case flat phase + REFCENTS")
-    w = W.C1
+    w = W.SCAO8
     c = Case(w.nsubx, w.n, w.nfft, w.nimg, obs=w.obs)
     c.run_both()
     ref = c.cents.copy()
@@ -60,7 +60,7 @@ def test_flat_phase_reference_and_refcents():
 @pytest.mark.parametrize("path", sorted(PATHS))
 def test_noise_poisson_read_relative_floor(path):
     """40x40 WFS, 1000 photons, Poisson + read noise (sigma 0.5) and a 0.1*max floor."""
-    w = W.C2
+    w = W.NOISE40
     say("case noise %s %s" % (w.name, path))
     c = Case(w.nsubx, w.n, w.nfft, w.nimg, obs=w.obs, sig=w.sig, readnoise=w.readnoise, addPoisson=1,
              noiseFloor=w.noiseFloor, threshType=2, phs=w.phase(), seed=5, form=PATHS[path])
@@ -207,7 +207,7 @@ def test_options(name, path):
 def test_integration_cycle():
     """nintegrations=2, nlatency=1: a readout every third call (cmod/src/centmodule.c:1774-1788)."""
     say("case integration cycle")
-    w = W.C1
+    w = W.SCAO8
     c = Case(w.nsubx, w.n, w.nfft, w.nimg, obs=w.obs, nintegrations=2, nlatency=1, phs=w.phase())
     outs = []
     for k in range(6):
@@ -221,7 +221,7 @@ def test_integration_cycle():
 
 def test_image_input_runslope():
     say("case runSlope image input")
-    w = W.C1
+    w = W.SCAO8
     src = Case(w.nsubx, w.n, w.nfft, w.nimg, obs=w.obs, phs=w.phase(), sig=100.)
     src.cc.run(0)
     img4 = src.bimg.copy()
@@ -241,7 +241,7 @@ def test_run_device_frames_match_legacy():
     import torch
     from dasp_amd.cmod import cent
     say("case run_device frames")
-    w = W.C1
+    w = W.SCAO8
     c = Case(w.nsubx, w.n, w.nfft, w.nimg, obs=w.obs)
     frames = np.stack([w.phase(f) for f in range(3)])
     want = []
@@ -267,7 +267,7 @@ def test_dft_mfma_form_error_vs_fp64(form):
     """The DFT-GEMM form (hi/lo split, FP32 accumulate) on the 80x80 nfft=32 WFS: RMS centroid
     error against FP64 within 1e-3 px (FP16 split is expected near the FP32 path)."""
     from dasp_amd.cmod import cent
-    w = W.C3
+    w = W.WFS80
     say("case DFT form %s" % form)
     c = Case(w.nsubx, w.n, w.nfft, w.nimg, obs=w.obs, phs=w.phase(),
              form={"fp16": cent.FORM_DFT_FP16, "bf16": cent.FORM_DFT_BF16}[form])
