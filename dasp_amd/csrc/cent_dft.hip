@@ -349,15 +349,21 @@ __global__ __launch_bounds__(256) void dft16_pipe_kernel(FusedArgs a_) {
     const int iA = s0 + 2 * p, iB = s0 + min(2 * p + 1, cnt - 1);
     const int rA = scalar_read(a.active, iA), rB = scalar_read(a.active, iB);
     const int oA = scalar_read(a.winoff, iA), oB = scalar_read(a.winoff, iB);
-    const int sA = vA ? rA : 0, sB = vB ? rB : 0;
-    const long wA = vA ? (long)f * a.phs_stride + oA : 0, wB = vB ? (long)f * a.phs_stride + oB : 0;
-    const float *ph = a.phs + (u ? wB : wA) + jj;
-    const float *pu = a.pup + (long)(u ? sB : sA) * (kN * kN) + jj;
+    // Item A's phase window and pupil are wave-uniform bases; the lanes of item B add B's
+    // distance from A (non-negative: active indices and window offsets ascend), so each load is
+    // a scalar base plus one 32-bit lane offset, with no 64-bit address arithmetic per load.
+    const long wA = vA ? (long)f * a.phs_stride + oA : 0;
+    const int sA = vA ? rA : 0;
+    const unsigned dW = vB ? (unsigned)(oB - oA) : 0u, dS = vB ? (unsigned)(rB - rA) : 0u;
+    const unsigned po = ((u ? dW : 0u) + (unsigned)(8 * h * a.npup + jj)) * 4u;
+    const unsigned uo = ((u ? dS * (kN * kN) : 0u) + (unsigned)(8 * h * kN + jj)) * 4u;
+    const char *ph = (const char *)(a.phs + wA);
+    const char *pu = (const char *)(a.pup + (long)sA * (kN * kN));
+    const long rstride = (long)a.npup * 4;
 #pragma unroll
     for (int t = 0; t < 8; ++t) {
-      const int i = 8 * h + t;
-      ph8[t] = ph[(long)i * a.npup];
-      pu8[t] = pu[i * kN];
+      ph8[t] = load_at(uniform_global(ph + t * rstride), po);
+      pu8[t] = *(const float *)(pu + t * (kN * 4) + uo);
     }
   };
   auto phase_factor = [&](const float (&ph8)[8], const float (&pu8)[8], COp<T8> &A1) {
@@ -475,9 +481,9 @@ __global__ __launch_bounds__(256) void dft16_pipe_kernel(FusedArgs a_) {
 #pragma unroll
       for (int sub = 0; sub < 2; ++sub) {
         if (sub < nvalid) {
-          float *bout = a.bimg + (long)f * a.bimg_stride + (long)sidx[sub] * (kM * kM);
+          char *bout = (char *)(a.bimg + (long)f * a.bimg_stride + (long)sidx[sub] * (kM * kM));
 #pragma unroll
-          for (int qq = 0; qq < 4; ++qq) bout[I * kM + pJ[qq]] = img[sub][qq];
+          for (int qq = 0; qq < 4; ++qq) *(float *)(bout + (unsigned)((I * kM + pJ[qq]) * 4)) = img[sub][qq];
         }
       }
     }

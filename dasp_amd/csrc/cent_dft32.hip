@@ -88,15 +88,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void d
     const float *p0 = a.phs + (v ? (long)f * a.phs_stride + scalar_read(a.winoff, s0 + r) : 0);
     const float *u0 = a.pup + (long)s * n * n;
     if constexpr (FULL) {
-      // wave-uniform row bases (scalar registers) plus one 32-bit lane offset
-      // 32-bit unsigned byte offsets (zero-extended), so that the loads take the scalar-base
-      // plus vector-offset form
-      const unsigned po = (unsigned)(8 * h * a.npup + c) * 4u, uo = (unsigned)(8 * h * 32 + c) * 4u;
+      // wave-uniform row bases (scalar registers) plus one 32-bit lane byte offset
+      unsigned po = (unsigned)(8 * h * a.npup + c) * 4u, uo = (unsigned)(8 * h * 32 + c) * 4u;
+      pin_offset(po);
+      pin_offset(uo);
+      GlobalBytes *ub = uniform_global(u0);
 #pragma unroll
       for (int q = 0; q < 16; ++q) {
         const int r = 16 * (q >> 3) + (q & 7);
-        ph[q] = *(const float *)((const char *)(p0 + (long)r * a.npup) + po);
-        pu[q] = *(const float *)((const char *)(u0 + r * 32) + uo);
+        ph[q] = load_at(uniform_global(p0 + (long)r * a.npup), po);
+        pu[q] = load_at(ub + r * 128, uo);
       }
     } else {
       // the zero factor is a product, not a select, so that the load is not made conditional

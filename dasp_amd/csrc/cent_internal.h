@@ -15,6 +15,42 @@ __device__ __forceinline__ T scalar_read(const T *p, long i) {
   return ((CT *)p)[i];
 }
 
+// Global-memory view of a wave-uniform address, rebuilt from its first lane's value. Row bases
+// made with it stay in scalar registers, and a load at base + (32-bit lane offset) then takes the
+// scalar-base plus vector-offset form; left to itself the compiler chains consecutive row
+// addresses through 64-bit vector adds from the first row's per-lane address.
+typedef const __attribute__((address_space(1))) char GlobalBytes;
+__device__ __forceinline__ GlobalBytes *uniform_global(const void *p) {
+  const unsigned long long v = (unsigned long long)p;
+  const unsigned long long lo = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)v);
+  const unsigned long long hi = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)(v >> 32));
+  return (GlobalBytes *)((hi << 32) | lo);
+}
+// A lane offset that is loop-invariant would be zero-extended once, before the loop, and the
+// 64-bit value no longer matches the scalar-base load form; an empty asm in the loop body keeps
+// the extension beside the loads. Call once per item on the offsets used by load_at.
+__device__ __forceinline__ void pin_offset(unsigned &byte_offset) { asm volatile("" : "+v"(byte_offset)); }
+__device__ __forceinline__ float load_at(GlobalBytes *base, unsigned byte_offset) {
+  return *(const __attribute__((address_space(1))) float *)(base + byte_offset);
+}
+
+// exp(i th) for the phase argument. Accurate path: sincosf. Fast path: th in revolutions reduced
+// to |r| <= 1/2 as r = th*c_hi - k + th*c_lo (c_hi + c_lo = 1/(2 pi) to 2^-48, k = rint(th*c_hi);
+// the first product is exact inside the fma), then the hardware sine/cosine of r revolutions.
+// Reduction error at most 2.6e-7 rad for |th| <= 400.
+template <bool FAST>
+__device__ __forceinline__ void phase_sincos(float th, float *sn, float *cs) {
+  if constexpr (FAST) {
+    const float k = rintf(th * 0.15915494309189535f);
+    float r = fmaf(th, 0.15915494309189535f, -k);
+    r = fmaf(th, 6.4206382e-09f, r);
+    *sn = __builtin_amdgcn_sinf(r);
+    *cs = __builtin_amdgcn_cosf(r);
+  } else {
+    sincosf(th, sn, cs);
+  }
+}
+
 // Image formation: phase -> |FFT|^2 placed as the reference's hll (cmod/src/centmodule.c:473-509).
 struct FormArgs {
   const float *phs;      // frame 0 phase (npup x npup)
@@ -125,7 +161,7 @@ struct FusedArgs {
   const float *pup;      // (nsubaps, n, n)
   const float *tilt;     // (n, n)
   float tiltstep;        // the reference's float tmp = pi/F*(F+1): tilt[i][j] = tmp*(i+j+1-n)
-  int fast_trig;         // 1: Cody-Waite reduction + hardware sin/cos instead of sincosf
+  int fast_trig;         // 1: reduction to revolutions + hardware sin/cos instead of sincosf
   int slow_detector;     // any of noise, sky array, relative floor, power, weights
   const int *active;
   const int *winoff;     // per active index: (sy*n)*npup + sx*n, the phase window's offset in a frame

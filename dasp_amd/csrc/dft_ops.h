@@ -152,16 +152,7 @@ __device__ __forceinline__ void cgemm_cv(const CConst<T8> &A, const COp<T8> &B, 
 
 template <bool FAST>
 __device__ __forceinline__ void dft_sincos(float th, float *sn, float *cs) {
-  if constexpr (FAST) {
-    const float k = rintf(th * 0.15915494309189535f);
-    float rem = fmaf(-k, 6.28318548202514648f, th);
-    rem = fmaf(-k, -1.7484556000744883e-07f, rem);
-    const float r = rem * 0.15915494309189535f;
-    *sn = __builtin_amdgcn_sinf(r);
-    *cs = __builtin_amdgcn_cosf(r);
-  } else {
-    sincosf(th, sn, cs);
-  }
+  phase_sincos<FAST>(th, sn, cs);
 }
 
 }  // namespace dasp

@@ -35,22 +35,6 @@ __device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// exp(i th) for the phase argument. Accurate path: sincosf. Fast path: Cody-Waite reduction by
-// 2 pi (two-constant split) to |rem| <= pi, then the hardware sine/cosine of rem/(2 pi) revolutions.
-template <bool FAST>
-__device__ __forceinline__ void phase_sincos(float th, float *sn, float *cs) {
-  if constexpr (FAST) {
-    const float k = rintf(th * 0.15915494309189535f);
-    float rem = fmaf(-k, 6.28318548202514648f, th);
-    rem = fmaf(-k, -1.7484556000744883e-07f, rem);
-    const float r = rem * 0.15915494309189535f;
-    *sn = __builtin_amdgcn_sinf(r);
-    *cs = __builtin_amdgcn_cosf(r);
-  } else {
-    sincosf(th, sn, cs);
-  }
-}
-
 template <int NP>
 struct PhaseCol {
   float ph[NP], pu[NP];
