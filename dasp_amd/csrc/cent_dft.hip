@@ -285,22 +285,36 @@ __global__ __launch_bounds__(256) void dft16_cog_kernel(FusedArgs a_) {
 // Each iteration is one basic block up to its stores (thresholds by selects, the second item of a
 // pair computed even when the slice has none and then not stored), so the scheduler interleaves
 // the matrix-core instructions with independent vector instructions of the same wave.
-template <typename T8, bool FAST>
-__global__ __launch_bounds__(256) void dft16_pipe_kernel(FusedArgs a_) {
+//
+// The half-pixel tilt is folded into the constant operands: exp(-i t) with
+// t = tiltstep*(i + j + 1 - n) factors into exp(-i tiltstep*i) on stage 1's contraction index and
+// exp(-i tiltstep*(j + 1 - n)) on stage 2's, so the phase factor is pup*exp(i phi) and the tilt
+// costs no vector instruction or register per pixel (the products are formed in FP64 before the
+// hi/lo split; the reference subtracts the float tilt from the phase, which rounds the argument).
+// FULLWIN (ncen == nimg): the CoG window is the whole detector, so no per-pixel window weights.
+template <typename T8, bool FAST, bool FULLWIN>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(FAST && FULLWIN ? 3 : 2)))
+void dft16_pipe_kernel(FusedArgs a_) {
   const FusedArgs &a = a_;
   const int l = threadIdx.x & 63;
   const int h = l >> 5;
   const int c = l & 31;
   const int jj = c & 15;
-  CConst<T8> B1, A2;   // as in dft16_cog_kernel
+  CConst<T8> B1, A2;   // as in dft16_cog_kernel, times the tilt factors
   {
     float re[8], im[8], re2[8], im2[8];
+    const double ts = (double)a.tiltstep, twopi_f = 6.283185307179586 / kF;
 #pragma unroll
     for (int t = 0; t < 8; ++t) {
       const int i = 8 * h + t;
-      sincospif(-2.0f * (float)((c * i) % kF) / kF, &im[t], &re[t]);
+      double sn, cs;
+      sincos(-twopi_f * (double)((c * i) % kF) - ts * (double)i, &sn, &cs);
+      re[t] = (float)cs;
+      im[t] = (float)sn;
       const int j = 8 * (t / 4) + 4 * h + t % 4;
-      sincospif(-2.0f * (float)((c * j) % kF) / kF, &im2[t], &re2[t]);
+      sincos(-twopi_f * (double)((c * j) % kF) - ts * (double)(j + 1 - a.n), &sn, &cs);
+      re2[t] = (float)cs;
+      im2[t] = (float)sn;
     }
     B1.set(re, im);
     A2.set(re2, im2);
@@ -313,21 +327,21 @@ __global__ __launch_bounds__(256) void dft16_pipe_kernel(FusedArgs a_) {
   const long wave_id = (long)(blockIdx.x / kXcds) * (blockDim.x >> 6) + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const long nwaves = (long)(gridDim.x / kXcds) * (blockDim.x >> 6);
   if (wave_id >= npairs) return;   // wave-uniform; no barrier in this kernel
-  const int n = a.n;
-  // detector constants of this lane (see dft16_cog_kernel)
+  // detector constants of this lane (see dft16_cog_kernel): row I, and columns J0 - d with
+  // d = 0, 1, 4, 5 for its four pixels
   const int I = (kF - 2 - (c & ~1)) >> 1;
   const float fI = (float)I;
   const int par = c & 1;
-  int pJ[4];
+  const int J0 = kM - 1 - 8 * par - 2 * h;
+  const float fJ0 = (float)J0;
+  constexpr int kD[4] = {0, 1, 4, 5};
   float wc[4], wj[4];
-  {
+  if constexpr (!FULLWIN) {
     const int st0 = (kM - a.ncen) / 2, en0 = kM - st0;
 #pragma unroll
     for (int qq = 0; qq < 4; ++qq) {
-      const int q = 4 * par + qq;
-      const int J = (kF - 2 - (8 * (q >> 1) + 4 * h + 2 * (q & 1))) >> 1;
+      const int J = J0 - kD[qq];
       const bool in = I >= st0 && I < en0 && J >= st0 && J < en0;
-      pJ[qq] = J;
       wc[qq] = in ? 1.f : 0.f;
       wj[qq] = in ? (float)J : 0.f;
     }
@@ -370,10 +384,8 @@ __global__ __launch_bounds__(256) void dft16_pipe_kernel(FusedArgs a_) {
     float er[8], ei[8];
 #pragma unroll
     for (int t = 0; t < 8; ++t) {
-      const int i = 8 * h + t;
-      const float th = ph8[t] - a.tiltstep * (float)(i + jj + 1 - n);   // float ops as the reference
       float sn, cs;
-      dft_sincos<FAST>(th, &sn, &cs);
+      dft_sincos<FAST>(ph8[t], &sn, &cs);   // the tilt is in B1 and A2
       er[t] = pu8[t] * cs;
       ei[t] = pu8[t] * sn;
     }
@@ -456,8 +468,16 @@ __global__ __launch_bounds__(256) void dft16_pipe_kernel(FusedArgs a_) {
         float v = fmaf(par ? pix[4 + qq] : pix[qq], scale, 0.f) + add;
         v = (thr_on && v < thr) ? 0.f : v - tsub;
         img[sub][qq] = v;
-        sw = fmaf(wc[qq], v, sw);
-        sxw = fmaf(wj[qq], v, sxw);
+        if constexpr (!FULLWIN) {
+          sw = fmaf(wc[qq], v, sw);
+          sxw = fmaf(wj[qq], v, sxw);
+        }
+      }
+      if constexpr (FULLWIN) {
+        // sum of v * (J0 - d) = J0 * sum(v) - (v1 + 4 v2 + 5 v3)
+        const float(&v)[4] = img[sub];
+        sw = (v[0] + v[1]) + (v[2] + v[3]);
+        sxw = fmaf(fJ0, sw, -fmaf(5.f, v[3], fmaf(4.f, v[2], v[1])));
       }
       sums[sub][0] = sw;
       sums[sub][1] = sxw;
@@ -481,9 +501,10 @@ __global__ __launch_bounds__(256) void dft16_pipe_kernel(FusedArgs a_) {
 #pragma unroll
       for (int sub = 0; sub < 2; ++sub) {
         if (sub < nvalid) {
-          char *bout = (char *)(a.bimg + (long)f * a.bimg_stride + (long)sidx[sub] * (kM * kM));
+          char *bout = (char *)(a.bimg + (long)f * a.bimg_stride + (long)sidx[sub] * (kM * kM)) +
+                       (unsigned)((I * kM + J0) * 4);
 #pragma unroll
-          for (int qq = 0; qq < 4; ++qq) *(float *)(bout + (unsigned)((I * kM + pJ[qq]) * 4)) = img[sub][qq];
+          for (int qq = 0; qq < 4; ++qq) *(float *)(bout - 4 * kD[qq]) = img[sub][qq];
         }
       }
     }
@@ -511,6 +532,35 @@ __global__ __launch_bounds__(256) void dft16_pipe_kernel(FusedArgs a_) {
   }
 }
 
+// Persistent launch of the pipelined kernel: every wave builds the DFT-matrix operands once, so
+// the grid is the resident wave count (workgroups per CU from the occupancy query, times the CU
+// count) or the pair count if smaller, in whole multiples of the XCD count.
+template <typename T8, bool FAST, bool FULLWIN>
+hipError_t launch_pipe_t(const FusedArgs &a, long npairs, hipStream_t st) {
+  static int per_cu = 0, cus = 0;
+  if (per_cu == 0) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, dft16_pipe_kernel<T8, FAST, FULLWIN>, 256, 0) !=
+            hipSuccess || per_cu < 1 || cus < 1) {
+      per_cu = 2;
+      cus = 256;
+    }
+  }
+  long blocks = (npairs + 3) / 4;
+  const long cap = (long)cus * per_cu;
+  if (blocks > cap) blocks = cap;
+  blocks = (blocks + kXcds - 1) / kXcds * kXcds;   // every XCD gets the same number of workgroups
+  hipLaunchKernelGGL((dft16_pipe_kernel<T8, FAST, FULLWIN>), dim3((unsigned)blocks), dim3(256), 0, st, a);
+  return hipGetLastError();
+}
+
+template <typename T8, bool FAST>
+hipError_t launch_pipe(const FusedArgs &a, bool full, long npairs, hipStream_t st) {
+  return full ? launch_pipe_t<T8, FAST, true>(a, npairs, st) : launch_pipe_t<T8, FAST, false>(a, npairs, st);
+}
+
 bool dft_supported(int n, int F, int nimg) {
   return (n == kN && F == kF && nimg == kM) || dft32_supported(n, F, nimg);
 }
@@ -520,20 +570,19 @@ hipError_t launch_dft(const FusedArgs &a, int bf16, hipStream_t st) {
   if (a.n != kN) return launch_dft32(a, bf16, st);
   const long npairs = (a.nitems + 1) / 2 + kXcds;
   long blocks = (npairs + 3) / 4;
-  // persistent grid: every wave builds the DFT-matrix operands once (hundreds of instructions),
-  // so launch about the resident wave count (two waves per SIMD) and grid-stride
+  // persistent grid for the general-detector kernel: every wave builds the DFT-matrix operands
+  // once (hundreds of instructions), so launch its resident wave count (two waves per SIMD)
   const long cap = 256L * 2;
   if (blocks > cap) blocks = cap;
   blocks = (blocks + kXcds - 1) / kXcds * kXcds;   // every XCD gets the same number of workgroups
   if (!a.slow_detector) {
+    const bool full = a.ncen == kM;
     if (bf16) {
-      if (a.fast_trig) hipLaunchKernelGGL((dft16_pipe_kernel<b8, true>), dim3((unsigned)blocks), dim3(256), 0, st, a);
-      else hipLaunchKernelGGL((dft16_pipe_kernel<b8, false>), dim3((unsigned)blocks), dim3(256), 0, st, a);
-    } else {
-      if (a.fast_trig) hipLaunchKernelGGL((dft16_pipe_kernel<h8, true>), dim3((unsigned)blocks), dim3(256), 0, st, a);
-      else hipLaunchKernelGGL((dft16_pipe_kernel<h8, false>), dim3((unsigned)blocks), dim3(256), 0, st, a);
+      if (a.fast_trig) return launch_pipe<b8, true>(a, full, npairs, st);
+      return launch_pipe<b8, false>(a, full, npairs, st);
     }
-    return hipGetLastError();
+    if (a.fast_trig) return launch_pipe<h8, true>(a, full, npairs, st);
+    return launch_pipe<h8, false>(a, full, npairs, st);
   }
   if (bf16) {
     if (a.fast_trig) hipLaunchKernelGGL((dft16_cog_kernel<b8, true>), dim3((unsigned)blocks), dim3(256), 0, st, a);
