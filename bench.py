@@ -352,7 +352,7 @@ def bench_multi(args, dev):
         "value": per_step / (ms / 1e3), "unit": "subap-images/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-        "config": {"workload": "c4_multi_wfs", "frames_per_step": frames, "subap_images_per_step": per_step,
+        "config": {"workload": "multi_wfs", "frames_per_step": frames, "subap_images_per_step": per_step,
                    "launches": launches, "wfs_outside_persistent_launches": others,
                    "fast_trig": not args.accurate_trig, "wall_s": wall},
     }

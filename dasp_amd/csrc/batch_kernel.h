@@ -1,11 +1,11 @@
 // batch_kernel.h - the multi-WFS persistent kernel, one instantiation per fused shape.
 //
 // The host lays the WFS of one shape out as a queue of groups (64/NP work items each, a WFS's
-// groups contiguous, items frame-major inside a WFS as in the single-WFS kernel). Waves walk the
-// queue in increasing order, statically or by atomic claims (WaveQueue); the next group's phase
-// and pupil are loaded while the current group is transformed, also across a WFS boundary. With
-// claims, the last wave to finish resets the counters, so a captured step graph replays the
-// launch without a memset node.
+// groups contiguous; items frame-major inside a WFS as in the single-WFS kernel, or
+// subaperture-major for spot kernels). Waves walk the queue in increasing order, statically or
+// by atomic claims (WaveQueue); the next group's phase and pupil are loaded while the current
+// group is transformed, also across a WFS boundary. With claims, the last wave to finish resets
+// the counters, so a captured step graph replays the launch without a memset node.
 //
 // One kernel per shape rather than one for all shapes: a kernel's register allocation covers
 // every shape compiled into it, and a five-shape kernel spills where each single-shape kernel
@@ -65,6 +65,14 @@ __device__ __forceinline__ long desc_end(const BatchArgs &b, int d) {
   return d + 1 < b.ndesc ? b.desc[d + 1].group0 : b.total_groups;
 }
 
+// Queue order inside a WFS: frame-major, or subaperture-major for WFS with spot kernels (a
+// group's items then share one subaperture's spot spectrum).
+template <bool SPOT>
+__device__ __forceinline__ ItemRef flat_item(const FusedArgs &a, long first, int sl) {
+  if constexpr (SPOT) return item_flat_by_subap(a, first, sl);
+  else return item_flat(a, first, sl);
+}
+
 template <int NP, int F, int B, int WAVES, bool FAST, bool SPOT>
 __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(NP <= 16 ? 2 : 1)))
 void batch_kernel(BatchArgs b, int chunk) {
@@ -81,7 +89,7 @@ void batch_kernel(BatchArgs b, int chunk) {
     int d = desc_of(b, g);
     long base = b.desc[d].group0, end = desc_end(b, d);
     PhaseCol<NP> cur;
-    ItemRef it = item_flat(desc_args(b, d), (g - base) * SPW, sl);
+    ItemRef it = flat_item<SPOT>(desc_args(b, d), (g - base) * SPW, sl);
     load_item<NP>(desc_args(b, d), it, j, cur);
     while (true) {
       const long gn = q.next(g);   // entries only grow: gn > g
@@ -90,7 +98,7 @@ void batch_kernel(BatchArgs b, int chunk) {
       PhaseCol<NP> nxt;
       ItemRef itn{0, 0, 0, false};
       if (more) {
-        itn = item_flat(desc_args(b, dn), (gn - b.desc[dn].group0) * SPW, sl);
+        itn = flat_item<SPOT>(desc_args(b, dn), (gn - b.desc[dn].group0) * SPW, sl);
         load_item<NP>(desc_args(b, dn), itn, j, nxt);
       }
       fused_item<NP, F, B, FAST, SPOT>(desc_args(b, d), it, j, w, sl, cur, st);

@@ -82,6 +82,23 @@ __device__ __forceinline__ ItemRef item_flat(const FusedArgs &a, long first, int
   return item_at(a, in ? f : -1, idx);
 }
 
+// Item first + sl of the subaperture-major numbering idx * nframes + frame, for the multi-WFS
+// spot kernels: the slices of a group are then consecutive frames of one subaperture and their
+// loads of its spot-spectrum rows and pupil hit the same cache lines.
+__device__ __forceinline__ ItemRef item_flat_by_subap(const FusedArgs &a, long first, int sl) {
+  first = wave_uniform((unsigned long long)first);
+  const long nfr = a.nitems / a.nactive;
+  const long i0 = first / nfr;
+  int f = (int)(first - i0 * nfr) + sl;
+  int idx = (int)i0;
+  while (f >= nfr) {   // at most sl / nfr + 1 steps
+    f -= (int)nfr;
+    ++idx;
+  }
+  const bool in = first + sl < a.nitems;
+  return item_at(a, in ? f : -1, idx);
+}
+
 // Load phase column j (rows i < n) and the pupil of one work item. The loads are unconditional
 // (no per-element branches): with n < NP the padding rows/columns read a clamped in-window
 // address and the pupil factor is zeroed by a product, so E is exactly 0 there.
