@@ -262,21 +262,24 @@ def test_run_device_frames_match_legacy():
     c.free()
 
 
-@pytest.mark.parametrize("form", ["fp16", "bf16"])
+# form codes of tests/_cases.Case: FORM_DFT_FP16 = 2, FORM_DFT_BF16 = 1; + 10 selects the reduced
+# hardware sin/cos (the library's default trig)
+DFT_FORMS = {"fp16": 2, "bf16": 1, "fp16_fasttrig": 12, "bf16_fasttrig": 11}
+
+
+@pytest.mark.parametrize("form", sorted(DFT_FORMS))
 def test_dft_mfma_form_error_vs_fp64(form):
     """The DFT-GEMM form (hi/lo split, FP32 accumulate) on the 80x80 nfft=32 WFS: RMS centroid
     error against FP64 within 1e-3 px (FP16 split is expected near the FP32 path)."""
-    from dasp_amd.cmod import cent
     w = W.WFS80
     say("case DFT form %s" % form)
-    c = Case(w.nsubx, w.n, w.nfft, w.nimg, obs=w.obs, phs=w.phase(),
-             form={"fp16": cent.FORM_DFT_FP16, "bf16": cent.FORM_DFT_BF16}[form])
+    c = Case(w.nsubx, w.n, w.nfft, w.nimg, obs=w.obs, phs=w.phase(), form=DFT_FORMS[form])
     oc, ob, _ = c.run_both()
     act = _active(c)
     d = report("dft-" + form, c.cents, oc, act)
     rms = np.sqrt((d ** 2).mean())
     assert rms < 1e-3
-    if form == "fp16":
+    if form.startswith("fp16"):
         assert d.max() < 1e-4
     c.free()
 
@@ -291,22 +294,20 @@ DFT16_CASES = {
 }
 
 
-@pytest.mark.parametrize("form", ["fp16", "bf16"])
+@pytest.mark.parametrize("form", sorted(DFT_FORMS))
 @pytest.mark.parametrize("name", sorted(DFT16_CASES))
 def test_dft16_options_vs_fp64(name, form):
     """The n=16/nfft=32 DFT-GEMM form with detector options, 11x11 WFS (an odd active count, so
     that XCD slices end in half-empty pairs), images and slopes against the FP64 oracle."""
-    from dasp_amd.cmod import cent
     say("case DFT16 %s %s" % (name, form))
     wgeo = W.WFS("dft16", 11, 16, 32, 16, 8.0, 0.2, 0.15, 25.0)
     kw = dict(DFT16_CASES[name])
-    c = Case(11, 16, 32, 16, obs=0.2, phs=wgeo.phase(),
-             form={"fp16": cent.FORM_DFT_FP16, "bf16": cent.FORM_DFT_BF16}[form], **kw)
+    c = Case(11, 16, 32, 16, obs=0.2, phs=wgeo.phase(), form=DFT_FORMS[form], **kw)
     oc, ob, _ = c.run_both()
     act = _active(c)
     say("active sub-apertures: %d" % act.sum())
     b = c.bimg.reshape(-1, 16, 16)[act]
-    tol = 5e-5 if form == "fp16" else 5e-4
+    tol = 5e-5 if form.startswith("fp16") else 5e-4
     # a pixel within the form's error of a threshold may fall on the other side of it
     diffpix = np.abs(b - ob[act]) > tol * np.abs(ob[act]).max()
     say("pixels beyond %.0e of the maximum: %d of %d" % (tol, diffpix.sum(), diffpix.size))
@@ -314,7 +315,7 @@ def test_dft16_options_vs_fp64(name, form):
     clean = ~diffpix.reshape(diffpix.shape[0], -1).any(axis=1)
     d = np.abs(c.cents.reshape(-1, 2)[act] - oc[act])[clean]
     assert np.sqrt((d ** 2).mean()) < 1e-3
-    if form == "fp16" and "noise" not in name:
+    if form.startswith("fp16") and "noise" not in name:
         assert d.max() < 1e-4
     assert np.all(c.cents.reshape(-1, 2)[~act] == 0)
     c.free()
@@ -345,17 +346,16 @@ DFT32_CASES = {
 }
 
 
-@pytest.mark.parametrize("form", ["fp16", "bf16"])
+@pytest.mark.parametrize("form", sorted(DFT_FORMS))
 @pytest.mark.parametrize("name", sorted(DFT32_CASES))
 def test_dft32_mfma_form_error_vs_fp64(name, form):
     """The DFT-GEMM form for phase size n <= 32, fftsize 64, nimg 16 (4x4 binning) against FP64:
     noiseless centroids to 1e-4 px (FP16 split) / RMS 1e-3 px (BF16 split); the images the
     correlation and fit stage reads within 5e-5 / 5e-4 of the image maximum."""
-    from dasp_amd.cmod import cent
     nsubx, n, corr, noise = DFT32_CASES[name]
     say("case DFT32 %s %s" % (name, form))
     wgeo = W.WFS("dft32", nsubx, n, 64, 16, 8.0, 0.2, 0.15, 25.0)
-    kw = dict(obs=0.2, phs=wgeo.phase(), form={"fp16": cent.FORM_DFT_FP16, "bf16": cent.FORM_DFT_BF16}[form])
+    kw = dict(obs=0.2, phs=wgeo.phase(), form=DFT_FORMS[form])
     if corr:
         kw.update(correlationCentroiding=1, corrPattern=W.reference_images(wgeo), parabolicFit=5)
     if noise:
@@ -375,9 +375,9 @@ def test_dft32_mfma_form_error_vs_fp64(name, form):
         return
     d = report("dft32-%s-%s" % (name, form), c.cents, oc, act)
     assert np.sqrt((d ** 2).mean()) < 1e-3
-    if form == "fp16":
+    if form.startswith("fp16"):
         assert d.max() < 1e-4
-    tol = 5e-5 if form == "fp16" else 5e-4
+    tol = 5e-5 if form.startswith("fp16") else 5e-4
     np.testing.assert_allclose(b, ob[act], rtol=0, atol=tol * ob[act].max())
     assert np.all(c.cents.reshape(-1, 2)[~act] == 0)
     c.free()
