@@ -292,8 +292,16 @@ __global__ __launch_bounds__(256) void dft16_cog_kernel(FusedArgs a_) {
 // costs no vector instruction or register per pixel (the products are formed in FP64 before the
 // hi/lo split; the reference subtracts the float tilt from the phase, which rounds the argument).
 // FULLWIN (ncen == nimg): the CoG window is the whole detector, so no per-pixel window weights.
+// Register budget: the FP16 kernel with fast trig and the full window fits three waves per SIMD
+// (163 VGPRs) with one wave_total per sum; the transposing six-value sum saves 40 vector
+// instructions per pair but needs 171 VGPRs (two waves). Measured on the 80x80 nfft32 workload,
+// the first is 1-2 % faster for FP16 and the second 2-3 % faster for BF16 (whose split code makes
+// it a two-wave kernel either way), so each type takes its faster variant.
 template <typename T8, bool FAST, bool FULLWIN>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(FAST && FULLWIN ? 3 : 2)))
+constexpr bool kPipeThreeWaves = std::is_same<T8, h8>::value && FAST && FULLWIN;
+
+template <typename T8, bool FAST, bool FULLWIN>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kPipeThreeWaves<T8, FAST, FULLWIN> ? 3 : 1)))
 void dft16_pipe_kernel(FusedArgs a_) {
   const FusedArgs &a = a_;
   const int l = threadIdx.x & 63;
@@ -487,10 +495,14 @@ void dft16_pipe_kernel(FusedArgs a_) {
     __builtin_amdgcn_sched_barrier(0);
     // ---- region 3 ----
     detector(1, xr1, xi1);
+    if constexpr (kPipeThreeWaves<T8, FAST, FULLWIN>) {
 #pragma unroll
-    for (int sub = 0; sub < 2; ++sub)
+      for (int sub = 0; sub < 2; ++sub)
 #pragma unroll
-      for (int k = 0; k < 3; ++k) sums[sub][k] = wave_total(sums[sub][k]);
+        for (int k = 0; k < 3; ++k) sums[sub][k] = wave_total(sums[sub][k]);
+    } else {
+      wave_totals6(sums[0], sums[1]);
+    }
     cf = nf;
     cp = np;
     advance(nf, np);
