@@ -86,6 +86,7 @@ def library():
         lib.dasp_cent_set_form.argtypes = [vp, ip]
         lib.dasp_cent_run_device.argtypes = [vp, vp, ctypes.c_long, ip, vp, vp, vp, vp, vp, vp]
         lib.dasp_cent_draw_normals.argtypes = [ctypes.c_ulong, ctypes.c_ulong, ctypes.c_long, vp]
+        lib.dasp_cent_draw_normals_device.argtypes = [ctypes.c_ulong, ctypes.c_ulong, ctypes.c_long, vp, vp]
         lib.dasp_cent_hc_to_spectrum.argtypes = [vp, ip, vp]
         lib.dasp_cent_info.argtypes = [vp, ctypes.POINTER(ip), ctypes.POINTER(ip), ctypes.POINTER(ip)]
         lib.dasp_cent_set_timing.argtypes = [vp, ip]
@@ -109,7 +110,8 @@ def library():
         for name in ("dasp_cent_initialise", "dasp_cent_run", "dasp_cent_run_slope", "dasp_cent_update_int",
                      "dasp_cent_update_float", "dasp_cent_update_array", "dasp_cent_update_caldata",
                      "dasp_cent_free", "dasp_cent_testcorrelation", "dasp_cent_set_form",
-                     "dasp_cent_run_device", "dasp_cent_draw_normals", "dasp_cent_hc_to_spectrum",
+                     "dasp_cent_run_device", "dasp_cent_draw_normals", "dasp_cent_draw_normals_device",
+                     "dasp_cent_hc_to_spectrum",
                      "dasp_cent_info", "dasp_cent_set_timing", "dasp_cent_kernel_time",
                      "dasp_cent_set_fast_trig"):
             getattr(lib, name).restype = ctypes.c_int
@@ -331,6 +333,13 @@ def drawNormals(seed, stream, n):
     out = numpy.empty(int(n), numpy.float32)
     _check(library().dasp_cent_draw_normals(int(seed), int(stream), int(n), out.ctypes.data))
     return out
+
+
+def drawNormalsDevice(seed, stream, n, out_ptr, hip_stream=0):
+    """The same stream written by the device into n floats at device address out_ptr,
+    asynchronously on hip_stream (values equal drawNormals' up to the last float bit)."""
+    _check(library().dasp_cent_draw_normals_device(int(seed), int(stream), int(n), int(out_ptr),
+                                                   int(hip_stream)))
 
 
 def hcToSpectrum(pattern_hc):

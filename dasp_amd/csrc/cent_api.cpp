@@ -129,7 +129,6 @@ struct dasp_cent {
   DevBuf d_otfk;   // spot kernel table of the fused kernels (psfsize == fftsize)
   DevBuf d_interp; // resampled phase grids when the input grid is not nsubx*phasesize
   std::vector<float> h_stage;
-  std::vector<float> noiseHost;
   // kernel timing
   bool timing = false;
   std::vector<hipEvent_t> evs;   // start/stop pairs not yet harvested
@@ -639,18 +638,18 @@ static int launch_phase(dasp_cent *c, const float *d_phs, long phs_stride, int n
   return 0;
 }
 
-// Pre-draw the readout's noise on the host and copy it to the device.
+// Draw the readout's noise streams (seed, 2k) and (seed, 2k+1) on the device, in stream order
+// before the image kernels (the host generator's sequence, without a host round trip).
 static int draw_noise(dasp_cent *c, bool pois, bool rd) {
   const long per = (long)c->nsubaps * c->nimg * c->nimg;
   const unsigned long k = c->nreadouts;
-  c->noiseHost.resize(per);
   if (pois) {
-    host_draw_normals(c->seed, 2 * k, per, c->noiseHost.data());
-    if (upload(c->d_zpois, c->noiseHost.data(), per * 4)) return fail(DASP_CENT_EHIP, "noise upload failed");
+    HIPCHK(c->d_zpois.ensure(per * 4));
+    HIPCHK(launch_normals(c->seed, 2 * k, per, c->d_zpois.as<float>(), nullptr));
   }
   if (rd) {
-    host_draw_normals(c->seed, 2 * k + 1, per, c->noiseHost.data());
-    if (upload(c->d_zread, c->noiseHost.data(), per * 4)) return fail(DASP_CENT_EHIP, "noise upload failed");
+    HIPCHK(c->d_zread.ensure(per * 4));
+    HIPCHK(launch_normals(c->seed, 2 * k + 1, per, c->d_zread.as<float>(), nullptr));
   }
   return 0;
 }
@@ -925,6 +924,12 @@ int dasp_cent_abi_version(void) { return DASP_CENT_ABI_VERSION; }
 int dasp_cent_draw_normals(unsigned long seed, unsigned long stream, long n, float *out) {
   if (n < 0 || (n > 0 && !out)) return fail(DASP_CENT_EINVAL, "bad arguments");
   host_draw_normals(seed, stream, n, out);
+  return 0;
+}
+
+int dasp_cent_draw_normals_device(unsigned long seed, unsigned long stream, long n, float *d_out, void *hstream) {
+  if (n < 0 || (n > 0 && !d_out)) return fail(DASP_CENT_EINVAL, "draw_normals_device: bad size or null output");
+  HIPCHK(launch_normals(seed, stream, n, d_out, (hipStream_t)hstream));
   return 0;
 }
 

@@ -1,6 +1,7 @@
 // cent_internal.h - argument blocks shared by the host engine (cent_api.cpp) and the kernels.
 #pragma once
 #include <hip/hip_runtime.h>
+#include <stdint.h>
 
 namespace dasp {
 
@@ -225,6 +226,10 @@ struct InterpArgs {
   int nframes;
 };
 hipError_t launch_interp(const InterpArgs &a, hipStream_t st);
+
+// n standard normals of the (seed, stream) noise stream into device memory (cent_noise.hip),
+// the host generator's sequence
+hipError_t launch_normals(uint64_t seed, uint64_t stream, long n, float *out, hipStream_t st);
 
 bool corr_fit_supported(int n);
 hipError_t launch_corr_fit(const CorrArgs &a, int n, bool corr, hipStream_t st);

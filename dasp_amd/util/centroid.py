@@ -201,10 +201,11 @@ class centroid:
         pois = self.addPoisson == 1 and calsource == 0
         rd = self.readnoise != 0 and calsource == 0
         k = self._readouts
-        if pois:
-            self._zp.copy_(torch.from_numpy(cent.drawNormals(self.seed, 2 * k, m).reshape(self._zp.shape)))
+        st = torch.cuda.current_stream(self.phase.device).cuda_stream
+        if pois:   # the readout's noise streams, drawn on the device in stream order
+            cent.drawNormalsDevice(self.seed, 2 * k, m, self._zp.data_ptr(), st)
         if rd:
-            self._zr.copy_(torch.from_numpy(cent.drawNormals(self.seed, 2 * k + 1, m).reshape(self._zr.shape)))
+            cent.drawNormalsDevice(self.seed, 2 * k + 1, m, self._zr.data_ptr(), st)
         npup = self.nsubx * self.phasesize
         phase = self.phase if self.phase.is_contiguous() else self.phase.contiguous()
         cent.runDevice(h, phase.data_ptr(), npup * npup, 1, self._dev_cents.data_ptr(), self.cmodbimg.data_ptr(),

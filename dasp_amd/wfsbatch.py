@@ -51,14 +51,15 @@ class WFSBatch:
         self.batch = cent.batchCreate(items)
 
     def draw_noise(self, seed, readout=0):
-        """Fill the noise arrays with the engine's host streams (seed, 2*readout+f*2 ...)."""
+        """Fill the noise arrays with the engine's noise streams: WFS k, frame f draws streams
+        (seed + k, 2*(readout+f)) and (seed + k, 2*(readout+f) + 1), generated on the device."""
+        st = self.stream()
         for k, cc in enumerate(self.ccs):
             for f in range(self.nframes):
                 for arr, off in ((self.zpois[k], 0), (self.zread[k], 1)):
                     if arr is None:
                         continue
-                    z = cent.drawNormals(seed + k, 2 * (readout + f) + off, arr[f].numel())
-                    arr[f].copy_(self.torch.from_numpy(z.reshape(arr[f].shape)))
+                    cent.drawNormalsDevice(seed + k, 2 * (readout + f) + off, arr[f].numel(), arr[f].data_ptr(), st)
 
     def stream(self):
         return self.torch.cuda.current_stream(self.dev).cuda_stream

@@ -179,6 +179,12 @@ int dasp_cent_run_device(dasp_cent_t *h, const float *d_phs, long phs_stride, in
  * (O'Neill 2014) uniforms through the Box-Muller transform. Deterministic and host only. */
 int dasp_cent_draw_normals(unsigned long seed, unsigned long stream, long n, float *out);
 
+/* The same stream drawn on the device into d_out (n floats), asynchronously on stream (no
+ * counterpart in the reference, which draws its noise inside the module). Every value equals the
+ * host generator's except, rarely, in the last float bit (device vs host double-precision
+ * log/sqrt/cos/sin). dasp_cent_run draws its readout noise this way. */
+int dasp_cent_draw_normals_device(unsigned long seed, unsigned long stream, long n, float *d_out, void *stream_);
+
 /* Complex spectrum (interleaved re,im; C*C per pattern) equivalent to a half-complex correlation
  * pattern, i.e. the K with calcCorrelation(img) = IDFT(DFT(pad(img)) * K) (host only). */
 int dasp_cent_hc_to_spectrum(const float *pattern_hc, int C, float *spectrum);

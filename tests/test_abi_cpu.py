@@ -38,6 +38,22 @@ def test_noise_stream_is_deterministic_normal():
     assert abs((a ** 3).mean()) < 0.03 and abs((a ** 4).mean() - 3) < 0.05
 
 
+def test_noise_stream_known_answer():
+    """The generator is PCG32 XSH-RR 64/32 (O'Neill 2014): its published demo sequence for
+    initstate 42, initseq 54 starts 0xa15c02b7 0x7b47f409 0xba1d3330 0x83d2f293 0xbfa4784b
+    0xcbed606e; consecutive outputs (u1, u2) = ((x + 0.5) / 2^32) give the Box-Muller pair
+    r cos(2 pi u2), r sin(2 pi u2), r = sqrt(-2 ln u1)."""
+    cent, _ = _lib()
+    x = np.array([0xa15c02b7, 0x7b47f409, 0xba1d3330, 0x83d2f293, 0xbfa4784b, 0xcbed606e], np.float64)
+    u = (x + 0.5) / 2.0 ** 32
+    r = np.sqrt(-2.0 * np.log(u[0::2]))
+    th = 2.0 * np.pi * u[1::2]
+    want = np.stack([r * np.cos(th), r * np.sin(th)], axis=1).ravel().astype(np.float32)
+    got = cent.drawNormals(42, 54, 6)
+    np.testing.assert_allclose(got, want, rtol=1e-6, atol=0)
+    np.testing.assert_array_equal(cent.drawNormals(42, 54, 5), got[:5])   # odd n: the last pair's cosine
+
+
 @pytest.mark.parametrize("C", [4, 6, 7, 8, 16, 32])
 def test_hc_to_spectrum_matches_oracle(C):
     cent, _ = _lib()

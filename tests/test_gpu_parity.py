@@ -398,3 +398,24 @@ def test_interpolated_phase_grid(m):
     b = c.bimg.reshape(-1, 8, 8)[act]
     np.testing.assert_allclose(b, ob[act], rtol=0, atol=2e-5 * ob[act].max())
     c.free()
+
+
+@pytest.mark.parametrize("seed,stream,n", [(1, 0, 1), (0, 1, 33), (7, 3, 200001), (11, 2 ** 40 + 5, 409600)])
+def test_device_noise_stream_matches_host(seed, stream, n):
+    """The device noise generator (dasp_cent_run's readout noise, WFSBatch.draw_noise) against the
+    host generator: the same PCG32 integers, so equal values except where the device and host
+    double-precision log/sqrt/cos/sin round to different floats (at most a few ulp, rarely)."""
+    import torch
+    from dasp_amd.cmod import cent
+    say("case device normals seed=%d stream=%d n=%d" % (seed, stream, n))
+    host = cent.drawNormals(seed, stream, n)
+    dev = torch.full((n + 1,), 123.0, dtype=torch.float32, device="cuda:0")
+    cent.drawNormalsDevice(seed, stream, n, dev.data_ptr(), torch.cuda.current_stream().cuda_stream)
+    torch.cuda.synchronize()
+    got = dev.cpu().numpy()
+    assert got[n] == 123.0   # nothing written past n
+    got = got[:n]
+    diff = got != host
+    say("values differing from the host: %d of %d" % (diff.sum(), n))
+    assert diff.sum() <= 2 + n // 100000
+    np.testing.assert_allclose(got, host, rtol=4 * np.finfo(np.float32).eps, atol=1e-30)
