@@ -539,12 +539,18 @@ static int launch_phase(dasp_cent *c, const float *d_phs, long phs_stride, int n
   const long nitems = (long)nframes * (long)c->active.size();
   const int np = pad_pow2(c->n);
   const float *amp = c->atmosPhaseType == 1 ? d_phs + (long)c->npup * c->npup : nullptr;
+  bool inactive_done = false;
   if (nitems > 0 && use_fused(c)) {
     FusedArgs fa;
     fused_args(c, d_phs, phs_stride, nframes, d_cents, d_bimg, d_zpois, d_zread, fa);
     const bool stage = corr_stage(c);
     if (stage && !d_bimg) return fail(DASP_CENT_EINVAL, "the correlation/fit stage needs the image buffer");
     fa.no_cents = stage;
+    if (!c->inactive.empty() && c->imageOnly == 0 && d_cents) {   // zeroed by the image kernel
+      fa.inactive = c->d_inactive.as<int>();
+      fa.ninactive = (int)c->inactive.size();
+      inactive_done = true;
+    }
     int trc = timing_mark(c, st);
     if (trc) return trc;
     if (use_dft(c)) HIPCHK(launch_dft(fa, c->form == DASP_CENT_FORM_DFT_BF16, st));
@@ -615,7 +621,7 @@ static int launch_phase(dasp_cent *c, const float *d_phs, long phs_stride, int n
     b.corrimg = d_corrimg;
     HIPCHK(launch_backend(b, st));
   }
-  if (!c->inactive.empty() && (c->imageOnly == 0 ? d_cents != nullptr : d_bimg != nullptr)) {
+  if (!inactive_done && !c->inactive.empty() && (c->imageOnly == 0 ? d_cents != nullptr : d_bimg != nullptr)) {
     InactiveArgs ia{};
     ia.inactive = c->d_inactive.as<int>();
     ia.ninactive = (int)c->inactive.size();

@@ -64,6 +64,7 @@ __global__ __launch_bounds__(256) void dft16_cog_kernel(FusedArgs a_) {
   const long npairs = cnt > 0 ? (a.nitems / a.nactive) * pp : 0;  // this XCD's pairs
   const long wave_id = (long)(blockIdx.x / kXcds) * (blockDim.x >> 6) + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const long nwaves = (long)(gridDim.x / kXcds) * (blockDim.x >> 6);
+  zero_inactive_slopes(a);
   if (wave_id >= npairs) return;   // wave-uniform; no barrier follows
   const int n = a.n;
   const int jj = c & 15;
@@ -334,6 +335,7 @@ void dft16_pipe_kernel(FusedArgs a_) {
   const long npairs = cnt > 0 ? (a.nitems / a.nactive) * pp : 0;
   const long wave_id = (long)(blockIdx.x / kXcds) * (blockDim.x >> 6) + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const long nwaves = (long)(gridDim.x / kXcds) * (blockDim.x >> 6);
+  zero_inactive_slopes(a);
   if (wave_id >= npairs) return;   // wave-uniform; no barrier in this kernel
   // detector constants of this lane (see dft16_cog_kernel): row I, and columns J0 - d with
   // d = 0, 1, 4, 5 for its four pixels

@@ -67,6 +67,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void d
   const long nx = cnt > 0 ? (a.nitems / a.nactive) * cnt : 0;   // this XCD's items
   const long wave_id = (long)(blockIdx.x / kXcds) * (blockDim.x >> 6) + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const long nwaves = (long)(gridDim.x / kXcds) * (blockDim.x >> 6);
+  zero_inactive_slopes(a);
   if (wave_id >= nx) return;   // wave-uniform; the workgroup has no barrier after this point
   const int n = a.n;
   // local item q = f * cnt + r (sub-aperture s0 + r of frame f), q advancing by nwaves: one

@@ -43,6 +43,7 @@ void fused_cog_kernel(FusedArgs a_) {
   const long ngx = nsgx * nframes;                           // this XCD's groups
   const long stride = (long)(gridDim.x / kXcds) * S::WAVES;
   long q = (long)(blockIdx.x / kXcds) * S::WAVES + w;
+  zero_inactive_slopes(a);
   if (q >= ngx) return;           // wave-uniform
   // group q = f * nsgx + g (group sg0 + g of frame f); q advances by a fixed stride, so frame
   // and group advance by fixed steps: one scalar division per wave

@@ -192,7 +192,26 @@ struct FusedArgs {
   const float2 *otfk;
   int otf_per_subap;
   int no_cents;          // slopes come from the correlation/fit stage (cent_corr.hip)
+  // inactive sub-apertures whose slopes the image kernel zeroes for every frame, or nullptr when
+  // the separate inactive kernel does it (multi-WFS step, image-only modes)
+  const int *inactive;
+  int ninactive;
 };
+
+// Slopes of the inactive sub-apertures set to zero for every frame (centroidsFromPhase gives
+// them cents = 0, cmod/src/centmodule.c:1964-1969), spread over all threads of the grid at the
+// start of an image kernel, so a step needs no second launch for them.
+__device__ __forceinline__ void zero_inactive_slopes(const FusedArgs &a) {
+  if (a.inactive == nullptr) return;
+  const long total = (a.nitems / a.nactive) * a.ninactive;
+  const long step = (long)gridDim.x * blockDim.x;
+  for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += step) {
+    const long f = e / a.ninactive;
+    float *co = a.cents + f * a.cents_stride + 2 * a.inactive[e - f * a.ninactive];
+    co[0] = 0.f;
+    co[1] = 0.f;
+  }
+}
 
 // Correlation / fit stage after the fused kernels (cent_corr.hip).
 struct CorrArgs {
