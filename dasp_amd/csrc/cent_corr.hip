@@ -64,28 +64,52 @@ __global__ __launch_bounds__(256) void corr_fit_kernel(CorrArgs a) {
   const int cnt = (int)((long)a.nactive * (xcd + 1) / kXcds) - s0;
   const int gpf = (cnt + IPW - 1) / IPW;                               // groups per frame
   const long ngroups = cnt > 0 ? (a.nitems / a.nactive) * gpf : 0;    // this XCD's groups
-  for (long g = (long)(blockIdx.x / kXcds) * WAVES + w; g < ngroups; g += (long)(gridDim.x / kXcds) * WAVES) {
-    const long gf = g / gpf;
-    const int rr = (int)(g - gf * gpf) * IPW + sl;
-    const bool valid = rr < cnt;
-    int frame = 0, s = 0;
-    if (valid) {
-      frame = (int)gf;
-      s = a.active[s0 + rr];
+  const long gstride = (long)(gridDim.x / kXcds) * WAVES;
+  const int M = a.nimg;
+  const int o = (N - M) / 2;   // centre padding when correlating (corrsize > nimg)
+  struct Item {
+    int frame, s;
+    bool valid;
+  };
+  auto item_of = [&](long g) {
+    Item it{0, 0, false};
+    if (g < ngroups) {
+      const long gf = g / gpf;
+      const int rr = (int)(g - gf * gpf) * IPW + sl;
+      if (rr < cnt) it = Item{(int)gf, a.active[s0 + rr], true};
     }
-    const int M = a.nimg;
+    return it;
+  };
+  // the lane's image row r (padded), read from a clamped in-bounds address and zeroed by a
+  // select, so the next group's loads are issued unconditionally while this group is transformed
+  auto load_row = [&](const Item &it, float (&v)[N]) {
+    const float *img = a.img + (long)it.frame * a.img_stride + (long)it.s * M * M;
+    const int ir = CORR ? r - o : r;
+    const bool rin = it.valid && ir >= 0 && ir < M;
+    const float *rp = img + (rin ? ir : 0) * M;
+    static_for<0, N>([&](auto cc) {
+      constexpr int c = decltype(cc)::value;
+      const int ic = CORR ? c - o : c;
+      const bool in = rin && ic >= 0 && ic < M;
+      const float val = rp[in ? ic : 0];
+      v[c] = in ? val : 0.f;
+    });
+  };
+  long g = (long)(blockIdx.x / kXcds) * WAVES + w;
+  Item cur = item_of(g);
+  float crow[N];
+  load_row(cur, crow);
+  for (; g < ngroups; g += gstride) {
+    const Item nxt = item_of(g + gstride);
+    float nrow[N];
+    load_row(nxt, nrow);
+    const bool valid = cur.valid;
+    const int frame = cur.frame, s = cur.s;
     const float *img = a.img + (long)frame * a.img_stride + (long)s * M * M;
     float row[N];
     if constexpr (CORR) {
-      // centre-pad to N, row r of the padded image
-      const int o = (N - M) / 2;
-      const int ir = r - o;
       cf x[N];
-      static_for<0, N>([&](auto cc) {
-        constexpr int c = decltype(cc)::value;
-        const bool in = valid && ir >= 0 && ir < M && c - o >= 0 && c - o < M;
-        x[c] = {in ? img[ir * M + c - o] : 0.f, 0.f};
-      });
+      static_for<0, N>([&](auto cc) { x[decltype(cc)::value] = {crow[decltype(cc)::value], 0.f}; });
       fft_fwd<N>(x);
       static_for<0, N>([&](auto v) { T[r * TS + decltype(v)::value] = x[decltype(v)::value]; });
       wave_sync();
@@ -117,7 +141,7 @@ __global__ __launch_bounds__(256) void corr_fit_kernel(CorrArgs a) {
         row[decltype(c)::value] = v < t ? 0.f : v - t;
       });
     } else {
-      static_for<0, N>([&](auto c) { row[decltype(c)::value] = valid ? img[r * N + decltype(c)::value] : 0.f; });
+      static_for<0, N>([&](auto c) { row[decltype(c)::value] = crow[decltype(c)::value]; });
     }
 
     // sum and first maximum over the whole image (computeFit :1087-1120; computeCoG :941-969)
@@ -253,6 +277,8 @@ __global__ __launch_bounds__(256) void corr_fit_kernel(CorrArgs a) {
       co[0] = cx;
       co[1] = cy;
     }
+    cur = nxt;
+    static_for<0, N>([&](auto c) { crow[decltype(c)::value] = nrow[decltype(c)::value]; });
   }
 }
 
