@@ -419,3 +419,43 @@ def test_device_noise_stream_matches_host(seed, stream, n):
     say("values differing from the host: %d of %d" % (diff.sum(), n))
     assert diff.sum() <= 2 + n // 100000
     np.testing.assert_allclose(got, host, rtol=4 * np.finfo(np.float32).eps, atol=1e-30)
+
+
+@pytest.mark.parametrize("form", ["fused", "dft16_fp16", "dft16_bf16_fasttrig", "dft32_fp16_corr"])
+def test_run_device_zeroes_inactive_slopes(form):
+    """The image kernels zero the inactive sub-apertures' slopes of every frame themselves (no
+    separate launch): an output buffer filled with garbage reads 0 there after a device run, and
+    the active slopes equal the legacy run's."""
+    import torch
+    from dasp_amd.cmod import cent
+    say("case inactive slopes %s" % form)
+    if form == "dft32_fp16_corr":
+        wgeo = W.WFS("dft32", 12, 32, 64, 16, 8.0, 0.2, 0.15, 25.0)
+        c = Case(12, 32, 64, 16, obs=0.2, phs=wgeo.phase(), form=2, correlationCentroiding=1,
+                 corrPattern=W.reference_images(wgeo), parabolicFit=5)
+    else:
+        wgeo = W.WFS("dft16", 11, 16, 32, 16, 8.0, 0.2, 0.15, 25.0)
+        code = {"fused": 10, "dft16_fp16": 2, "dft16_bf16_fasttrig": 11}[form]
+        c = Case(11, 16, 32, 16, obs=0.2, phs=wgeo.phase(), form=code)
+    nframes = 3
+    frames = np.stack([wgeo.phase(f) for f in range(nframes)])
+    want = []
+    for f in range(nframes):
+        c.phs[:] = frames[f]
+        c.cc.run(0)
+        want.append(c.cents.copy())
+    dev = torch.device("cuda:0")
+    ph = torch.from_numpy(frames).to(dev)
+    S = c.nsubx * c.nsubx
+    ce = torch.full((nframes, S, 2), -9.0, dtype=torch.float32, device=dev)
+    bi = torch.zeros((nframes, S, c.nimg, c.nimg), dtype=torch.float32, device=dev)
+    nact, npup, csz = cent.info(c.cc.centstruct)
+    ci = torch.zeros((nframes, S, csz, csz), dtype=torch.float32, device=dev)
+    cent.runDevice(c.cc.centstruct, ph.data_ptr(), npup * npup, nframes, ce.data_ptr(), bi.data_ptr(),
+                   corrimg_ptr=ci.data_ptr(), stream=torch.cuda.current_stream().cuda_stream)
+    torch.cuda.synchronize()
+    got = ce.cpu().numpy().reshape(nframes, -1, 2)
+    act = _active(c)
+    assert np.all(got[:, ~act] == 0)
+    np.testing.assert_array_equal(got[:, act], np.stack(want).reshape(nframes, -1, 2)[:, act])
+    c.free()
