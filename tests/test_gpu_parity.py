@@ -459,3 +459,34 @@ def test_run_device_zeroes_inactive_slopes(form):
     assert np.all(got[:, ~act] == 0)
     np.testing.assert_array_equal(got[:, act], np.stack(want).reshape(nframes, -1, 2)[:, act])
     c.free()
+
+
+@pytest.mark.parametrize("form", [10, 12], ids=["fused_fasttrig", "dft16_fp16_fasttrig"])
+def test_degenerate_geometries(form):
+    """Edge cases: every sub-aperture inactive (0.99 obstruction: slopes all 0, no image kernel),
+    a single sub-aperture (nsubx 1: seven of the eight XCD slices empty), and a device step of
+    zero frames (rejected with an error, buffers untouched)."""
+    import torch
+    from dasp_amd.cmod import cent
+    say("case degenerate geometries form %d" % form)
+    c = Case(2, 16, 32, 16, obs=0.99, form=form)
+    assert c.cc.subflag.sum() == 0
+    c.cents[:] = -9.0
+    oc, _, _ = c.run_both()
+    assert np.all(c.cents == 0) and np.all(oc == 0)
+    c.free()
+    w = W.WFS("one", 1, 16, 32, 16, 0.5, 0.0, 0.15, 25.0)
+    c = Case(1, 16, 32, 16, obs=0.0, phs=w.phase(), form=form)
+    oc, ob, _ = c.run_both()
+    act = _active(c)
+    assert act.sum() == 1
+    assert report("one subaperture", c.cents, oc, act).max() < 1e-4
+    dev = torch.device("cuda:0")
+    ph = torch.from_numpy(c.phs).to(dev)
+    ce = torch.full((1, 1, 2), -9.0, dtype=torch.float32, device=dev)
+    with pytest.raises(cent.error):
+        cent.runDevice(c.cc.centstruct, ph.data_ptr(), 256, 0, ce.data_ptr(),
+                       stream=torch.cuda.current_stream().cuda_stream)
+    torch.cuda.synchronize()
+    assert np.all(ce.cpu().numpy() == -9.0)
+    c.free()
