@@ -158,9 +158,10 @@ int dasp_cent_testcorrelation(const float *img, float *corr, int n);
  * no spot kernel; other geometries return DASP_CENT_EUNSUPPORTED. */
 int dasp_cent_set_form(dasp_cent_t *h, int form);
 
-/* Phase factor exp(i(phi - t)) in the fused kernels: 1 (default) = two-constant Cody-Waite
- * reduction by 2 pi and the hardware sine/cosine (v_sin_f32/v_cos_f32; max abs error 3.8e-7 on
- * |phi - t| <= 400 rad), 0 = sincosf (max abs error 7e-8). The general kernels use sincosf. */
+/* Phase factor exp(i(phi - t)) in the fused and DFT-GEMM kernels: 1 (default) = the argument
+ * reduced to revolutions, r = x*c_hi - k + x*c_lo with c_hi + c_lo = 1/(2 pi) (reduction error at
+ * most 2.6e-7 rad for |x| <= 400 rad), then the hardware sine/cosine (v_sin_f32/v_cos_f32);
+ * 0 = sincosf (max abs error 7e-8). The general kernels use sincosf. */
 int dasp_cent_set_fast_trig(dasp_cent_t *h, int enable);
 
 /* Device-resident step over nframes phase frames (science/wfscent.py:630-702 keeps phase,
