@@ -154,3 +154,53 @@ def test_batch_tune_and_update_rebuild():
     b.free()
     for c in cases:
         c.free()
+
+
+def test_batch_with_dft_form_members():
+    """WFS on the DFT-GEMM forms (n16/nfft32 FP16, n32/nfft64 BF16 with the correlation stage) run
+    their own kernels inside the step graph beside a fused member: each equals its eager per-WFS
+    device run, over graph replays with new phase written in place."""
+    import torch
+    from dasp_amd.cmod import cent
+    from dasp_amd.wfsbatch import WFSBatch
+    say("case batch with DFT-form members")
+    w16 = W.WFS("d16", 11, 16, 32, 16, 8.0, 0.2, 0.15, 25.0)
+    w32 = W.WFS("d32", 8, 32, 64, 16, 8.0, 0.2, 0.15, 25.0)
+    cases = [Case(11, 16, 32, 16, obs=0.2, form=12),
+             Case(8, 32, 64, 16, obs=0.2, form=11, correlationCentroiding=1,
+                  corrPattern=W.reference_images(w32), parabolicFit=5),
+             Case(12, 16, 32, 16, obs=0.2, form=10)]
+    frames = 2
+    b = WFSBatch([c.cc for c in cases], frames)
+    launches, others = b.info()
+    say("launches %s, others %d" % (launches, others))
+    assert others == 2
+    wgeo = [w16, w32, W.WFS("f16", 12, 16, 32, 16, 8.0, 0.2, 0.15, 25.0)]
+    for k in range(len(cases)):
+        for f in range(frames):
+            b.phs[k][f].copy_(torch.from_numpy(wgeo[k].phase(f)))
+    for rep in range(2):
+        for k in range(len(cases)):
+            b.phs[k].mul_(1.0 if rep == 0 else -0.8)
+        torch.cuda.synchronize()
+        for ce in b.cents:
+            ce.fill_(-7.0)
+        b.run()
+        torch.cuda.synchronize()
+        st = torch.cuda.current_stream().cuda_stream
+        for k, c in enumerate(cases):
+            ce = torch.full_like(b.cents[k], -9.0)
+            npup = c.nsubx * c.n
+            nact, _, csz = cent.info(c.cc.centstruct)
+            bi = torch.zeros((frames, c.nsubx * c.nsubx, c.nimg, c.nimg), dtype=torch.float32, device="cuda:0")
+            ci = torch.zeros((frames, c.nsubx * c.nsubx, csz, csz), dtype=torch.float32, device="cuda:0")
+            cent.runDevice(c.cc.centstruct, b.phs[k].data_ptr(), npup * npup, frames, ce.data_ptr(), bi.data_ptr(),
+                           ci.data_ptr(), stream=st)
+            torch.cuda.synchronize()
+            got, want = b.slopes(k), ce.cpu().numpy()
+            d = np.abs(got - want).max()
+            say("replay %d member %d: max|batch - per-WFS| = %.3g px" % (rep, k, d))
+            assert d <= 1e-5
+    b.free()
+    for c in cases:
+        c.free()
