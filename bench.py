@@ -162,6 +162,8 @@ def main():
                            0.5, 0, None, w.correlation, 0., corr, corrimg, w.threshType, 0, 0, 1, w.parabolicFit,
                            None)
     h = cc.centstruct
+    if w.relativeFloor:
+        cent.setRelativeFloor(h, True)
     cent.setForm(h, {"fft32": 0, "bf16": 1, "fp16": 2}[args.form])
     cent.setFastTrig(h, not args.accurate_trig)
     nact, npup, csz = cent.info(h)
@@ -318,6 +320,8 @@ def bench_multi(args, dev):
         cc = centcmod.centcmod(1, w.nsubx, w.cen, w.nfft, w.clip, w.nimg, w.n, w.readnoise, 0., w.addPoisson,
                                w.noiseFloor, float(w.sig), 0., 0, 1., 1, 0, 1 + k, host_phs, pupil, spots, cents_h,
                                bimg_h, 0.5, 0, None, 0, 0., None, None, w.threshType, 0, 0, 1, 0, None)
+        if w.relativeFloor:
+            cent_mod().setRelativeFloor(cc.centstruct, True)
         cent_mod().setFastTrig(cc.centstruct, not args.accurate_trig)
         ccs.append(cc)
         hosts.append((host_phs, cents_h, bimg_h))

@@ -1,8 +1,12 @@
-"""Build the gfx950 engine library ``dasp_amd/libdasp_cent.so`` with hipcc.
+"""Build the gfx950 engine: hipcc objects, the C-ABI library and the ``cent`` extension module.
 
-Mirrors how DASP builds its cmod extensions (object files linked into one shared library,
-cmod/src/Makefile.generic:12-22, cmod/src/setup.py:76-82), with hipcc instead of gcc and a plain
-C ABI instead of the Python 2 C-API. ``python -m dasp_amd.build`` rebuilds everything.
+1. every HIP/C++ source of the engine is compiled for gfx950 into ``dasp_amd/csrc/_build``;
+2. the objects are linked into ``dasp_amd/libdasp_cent.so``, the plain C ABI of
+   include/dasp_cent.h for callers in other languages;
+3. ``dasp_amd/setup.py build_ext --inplace`` builds the CPython extension ``cent``
+   (``dasp_amd/cmod/cent*.so``, PyInit_cent) over the same objects as ``extra_objects``, the way
+   DASP builds its cmod extensions (cmod/src/setup.py:55-63, :76-82).
+``python -m dasp_amd.build`` rebuilds everything.
 """
 import os
 import subprocess
@@ -54,6 +58,14 @@ def build(verbose=False):
     os.replace(tmp, LIB)
     if verbose:
         print("built", LIB)
+    root = os.path.dirname(HERE)
+    r = subprocess.run([sys.executable, os.path.join(HERE, "setup.py"), "build_ext", "--inplace", "--force",
+                        "--build-temp", os.path.join(OBJDIR, "ext"),
+                        "--build-lib", os.path.join(OBJDIR, "ext", "lib")], cwd=root, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("extension build failed:\n" + (r.stdout + r.stderr)[-6000:])
+    if verbose:
+        print("built the cent extension module in", os.path.join(HERE, "cmod"))
     return LIB
 
 

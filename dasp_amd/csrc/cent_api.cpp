@@ -13,6 +13,7 @@
 
 #include <cmath>
 #include <cstdarg>
+#include <cstddef>
 #include <cstdio>
 #include <cstring>
 #include <algorithm>
@@ -25,6 +26,15 @@
 #include "host_math.h"
 
 using namespace dasp;
+
+// Layout pins of the ABI structures (ctypes in dasp_amd/cmod/cent.py and the extension module
+// mirror them); a change must bump DASP_CENT_ABI_VERSION and these numbers together.
+static_assert(DASP_CENT_ABI_VERSION == 2, "ABI version and layout pins out of step");
+static_assert(sizeof(dasp_cent_array) == 48, "dasp_cent_array layout changed");
+static_assert(sizeof(dasp_cent_params) == 856, "dasp_cent_params layout changed");
+static_assert(offsetof(dasp_cent_params, seed) == 168 && offsetof(dasp_cent_params, phs) == 176 &&
+              offsetof(dasp_cent_params, subapLocation) == 808, "dasp_cent_params field offsets changed");
+static_assert(sizeof(dasp_cent_batch_item) == 72, "dasp_cent_batch_item layout changed");
 
 namespace {
 
@@ -112,6 +122,8 @@ struct dasp_cent {
   bool haveSigArr = false, haveSkyArr = false, haveRef = false, haveCalData = false;
   bool haveCalCoeff = false, haveUBArr = false, haveCorr = false, haveCentWeight = false;
   bool tablesDirty = true;
+  bool relFloor = false;   // engine option: threshType 0 floor taken relative to the image maximum
+  unsigned long scratchEpoch = 0;   // bumped whenever a scratch buffer of the handle moves
   std::vector<int> active, inactive, subflag;
   std::vector<float> pupfn, frac, sigArr, tilt, totsig, spot;
   int fitMxSize = -999;
@@ -119,6 +131,7 @@ struct dasp_cent {
   // borrowed host outputs / inputs
   float *h_phs = nullptr, *h_cents = nullptr, *h_bimg = nullptr, *h_corrimg = nullptr;
   const float *h_input = nullptr;
+  float *h_outphase = nullptr;   // OUTPUTPHASEARR: receives the interpolated phase of active windows
   int inputDims = 0, inputNpxlx = 0;
   long phsElems = 0;
   // device state
@@ -135,6 +148,16 @@ struct dasp_cent {
   double kernel_ms = 0.0;
   long kernel_launches = 0;
 };
+
+// Grows a scratch buffer of the handle. When the buffer moves (a larger allocation replaces it),
+// the handle's scratch epoch is bumped: a multi-WFS step graph that captured the old address is
+// stale and is rebuilt at its next run.
+static hipError_t grow_scratch(dasp_cent *c, DevBuf &b, size_t nb) {
+  const void *old = b.p;
+  const hipError_t e = b.ensure(nb);
+  if (b.p != old) c->scratchEpoch++;
+  return e;
+}
 
 static int harvest_events(dasp_cent *c) {
   for (size_t i = 0; i + 1 < c->evs.size(); i += 2) {
@@ -359,6 +382,16 @@ static int set_cent_weight(dasp_cent *c, const dasp_cent_array &a) {
 
 static bool check_f32(const dasp_cent_array &a) { return a.data != nullptr; }
 
+// Threshold code the kernels apply: 0 and 1 as readCCD (cmod/src/centmodule.c:785-817); 2 the
+// relative floor (the engine option of dasp_cent_set_relative_floor, on top of threshType 0);
+// -1 for every other threshType, which readCCD treats as unknown (:797, :813): no read noise,
+// no readBg and no floor.
+static int kthresh(const dasp_cent *c) {
+  if (c->threshType == 0) return c->relFloor ? 2 : 0;
+  return c->threshType == 1 ? 1 : -1;
+}
+
+
 // ------------------------------------------------------------------------------------------------
 // kernel argument assembly and launch
 // ------------------------------------------------------------------------------------------------
@@ -394,7 +427,7 @@ static void fill_back(dasp_cent *c, BackArgs &b, int nframes) {
   b.readnoise = c->calsource == 0 ? c->readnoise : 0.f;
   b.readbg = c->readbg;
   b.floor = c->noiseFloor;
-  b.threshType = c->threshType;
+  b.threshType = kthresh(c);
   b.opticalBinning = c->opticalBinning;
   b.calsource = c->calsource;
   b.useBrightest = c->useBrightest;
@@ -499,7 +532,7 @@ static void fused_args(dasp_cent *c, const float *d_phs, long phs_stride, int nf
   fa.readnoise = c->calsource == 0 ? c->readnoise : 0.f;
   fa.readbg = c->readbg;
   fa.floor = c->noiseFloor;
-  fa.threshType = c->threshType;
+  fa.threshType = kthresh(c);
   fa.calsource = c->calsource;
   fa.pxlPower = c->pxlPower;
   fa.centWeight = c->haveCentWeight ? c->d_centWeight.as<float>() : nullptr;
@@ -510,7 +543,7 @@ static void fused_args(dasp_cent *c, const float *d_phs, long phs_stride, int nf
   fa.bimg = d_bimg;
   fa.bimg_stride = (long)c->nsubaps * c->nimg * c->nimg;
   fa.slow_detector = (d_zpois && need_pois(c)) || (d_zread && need_read(c)) || c->haveSkyArr ||
-                     c->threshType == 2 || (c->pxlPower != 1.f && c->pxlPower != 0.f) || c->haveCentWeight;
+                     kthresh(c) == 2 || (c->pxlPower != 1.f && c->pxlPower != 0.f) || c->haveCentWeight;
   fa.otfk = c->spotDim ? c->d_otfk.as<float2>() : nullptr;
   fa.otf_per_subap = c->spotDim == 4;
 }
@@ -523,7 +556,7 @@ static int launch_phase(dasp_cent *c, const float *d_phs, long phs_stride, int n
   if (rc) return rc;
   if (c->interpolate) {   // interpolatePhase (:1628-1758) onto the nsubx*phasesize grid
     const long tot2 = (long)c->npup * c->npup;
-    HIPCHK(c->d_interp.ensure((size_t)nframes * tot2 * 4));
+    HIPCHK(grow_scratch(c, c->d_interp, (size_t)nframes * tot2 * 4));
     InterpArgs ia{};
     ia.in = d_phs;
     ia.in_stride = phs_stride;
@@ -588,7 +621,7 @@ static int launch_phase(dasp_cent *c, const float *d_phs, long phs_stride, int n
     }
   } else if (nitems > 0) {
     const long hll_elems = nitems * (long)c->padded * c->padded;
-    HIPCHK(c->d_hll.ensure(hll_elems * sizeof(float)));
+    HIPCHK(grow_scratch(c, c->d_hll, hll_elems * sizeof(float)));
     if (c->padded > c->F) HIPCHK(launch_fill(c->d_hll.as<float>(), hll_elems, 0.f, st));
     FormArgs fa{};
     fa.phs = d_phs;
@@ -632,7 +665,7 @@ static int launch_phase(dasp_cent *c, const float *d_phs, long phs_stride, int n
     ia.readnoise = c->calsource == 0 ? c->readnoise : 0.f;
     ia.readbg = c->readbg;
     ia.floor = c->noiseFloor;
-    ia.threshType = c->threshType;
+    ia.threshType = kthresh(c);
     ia.zread = d_zread;
     ia.noise_stride = (long)c->nsubaps * c->nimg * c->nimg;
     ia.cents = d_cents;
@@ -669,6 +702,16 @@ static double now_s() {
 // Copy device outputs into the borrowed host arrays with the reference's write pattern.
 static int copy_back(dasp_cent *c) {
   const int M2 = c->nimg * c->nimg;
+  if (c->interpolate && c->h_outphase) {   // interpolatePhase writes each active window (:1748-1755)
+    const size_t tot2 = (size_t)c->npup * c->npup;
+    c->h_stage.resize(tot2);
+    HIPCHK(hipMemcpy(c->h_stage.data(), c->d_interp.p, tot2 * 4, hipMemcpyDeviceToHost));
+    for (int s : c->active) {
+      const size_t o = (size_t)(s / c->nsubx) * c->n * c->npup + (size_t)(s % c->nsubx) * c->n;
+      for (int y = 0; y < c->n; ++y)
+        std::memcpy(c->h_outphase + o + (size_t)y * c->npup, c->h_stage.data() + o + (size_t)y * c->npup, c->n * 4);
+    }
+  }
   if (c->imageOnly == 0 && c->h_cents)
     HIPCHK(hipMemcpy(c->h_cents, c->d_cents.p, (size_t)c->nsubaps * 2 * 4, hipMemcpyDeviceToHost));
   if (c->h_bimg) {
@@ -710,17 +753,17 @@ static int check_device_run(dasp_cent *c, const float *d_phs, long phs_stride, i
 // Images and correlation images the general path materialises even when the caller wants none.
 static int device_scratch(dasp_cent *c, int nframes, float **bimg, float **corr) {
   if (!*bimg && (!use_fused(c) || corr_stage(c))) {
-    HIPCHK(c->d_bscratch.ensure((size_t)nframes * c->nsubaps * c->nimg * c->nimg * 4));
+    HIPCHK(grow_scratch(c, c->d_bscratch, (size_t)nframes * c->nsubaps * c->nimg * c->nimg * 4));
     *bimg = c->d_bscratch.as<float>();
   }
   if (!*corr && c->correlationCentroiding && c->haveCorr) {
-    HIPCHK(c->d_cscratch.ensure((size_t)nframes * c->nsubaps * c->corrsize * c->corrsize * 4));
+    HIPCHK(grow_scratch(c, c->d_cscratch, (size_t)nframes * c->nsubaps * c->corrsize * c->corrsize * 4));
     *corr = c->d_cscratch.as<float>();
   }
-  if (c->interpolate) HIPCHK(c->d_interp.ensure((size_t)nframes * c->npup * c->npup * 4));
+  if (c->interpolate) HIPCHK(grow_scratch(c, c->d_interp, (size_t)nframes * c->npup * c->npup * 4));
   if (!use_fused(c) && c->active.size() > 0) {   // image-formation scratch of the general path
     const long hll = (long)nframes * (long)c->active.size() * c->padded * c->padded;
-    HIPCHK(c->d_hll.ensure(hll * sizeof(float)));
+    HIPCHK(grow_scratch(c, c->d_hll, hll * sizeof(float)));
   }
   return 0;
 }
@@ -747,6 +790,7 @@ struct BatchLaunch {
 struct dasp_cent_batch {
   std::vector<dasp_cent_batch_item> items;
   std::vector<unsigned long> gens;   // handle generations the graph was built on
+  std::vector<unsigned long> epochs; // handle scratch epochs the graph was built on
   std::vector<BatchLaunch> launches;
   std::vector<int> others;           // items running their own kernels inside the graph
   DevBuf inactive;                   // InactiveDesc[] of the items in persistent launches
@@ -871,6 +915,9 @@ static int batch_layout(dasp_cent_batch *b) {
     int rc = device_scratch(it.handle, it.nframes, &bimg, &corr);
     if (rc) return rc;
   }
+  // scratch addresses are final here: the graph captured next uses exactly these
+  b->epochs.clear();
+  for (const dasp_cent_batch_item &it : b->items) b->epochs.push_back(it.handle->scratchEpoch);
   b->ninactive_desc = (int)ina.size();
   b->inactive_total = ina_total;
   if (!ina.empty() && upload(b->inactive, ina.data(), ina.size() * sizeof(InactiveDesc)))
@@ -914,11 +961,16 @@ static int batch_capture(dasp_cent_batch *b) {
 }
 
 static int batch_refresh(dasp_cent_batch *b) {
-  bool stale = !b->exec;
-  for (size_t i = 0; i < b->items.size(); ++i) stale = stale || b->items[i].handle->gen != b->gens[i];
+  const size_t n = b->items.size();
+  bool stale = !b->exec || b->gens.size() != n || b->epochs.size() != n;   // short after a failed layout
+  for (size_t i = 0; i < b->items.size() && !stale; ++i)
+    stale = b->items[i].handle->gen != b->gens[i] || b->items[i].handle->scratchEpoch != b->epochs[i];
   if (!stale) return 0;
   int rc = batch_layout(b);
-  if (rc) return rc;
+  if (rc) {
+    batch_drop_graph(b);   // never replay a graph whose layout could not be refreshed
+    return rc;
+  }
   return batch_capture(b);
 }
 
@@ -1161,6 +1213,14 @@ int dasp_cent_set_form(dasp_cent_t *c, int form) {
   return 0;
 }
 
+int dasp_cent_set_relative_floor(dasp_cent_t *c, int enable) {
+  if (!c) return fail(DASP_CENT_EINVAL, "null handle");
+  if (enable && c->threshType != 0) return fail(DASP_CENT_EINVAL, "the relative floor applies to threshType 0 (got %d)", c->threshType);
+  c->gen++;
+  c->relFloor = enable != 0;
+  return 0;
+}
+
 int dasp_cent_set_fast_trig(dasp_cent_t *c, int enable) {
   if (!c) return fail(DASP_CENT_EINVAL, "null handle");
   c->gen++;
@@ -1261,7 +1321,7 @@ int dasp_cent_run_slope(dasp_cent_t *c, float *seconds) {
     ia.readnoise = c->calsource == 0 ? c->readnoise : 0.f;
     ia.readbg = c->readbg;
     ia.floor = c->noiseFloor;
-    ia.threshType = c->threshType;
+    ia.threshType = kthresh(c);
     ia.zread = rd ? c->d_zread.as<float>() : nullptr;
     ia.noise_stride = nin;
     ia.cents = c->d_cents.as<float>();
@@ -1332,8 +1392,7 @@ int dasp_cent_update_float(dasp_cent_t *c, int code, double v) {
   c->gen++;
   switch (code) {
     case DASP_CENT_SIG:
-      c->sig = (float)v;
-      c->haveSigArr = false;
+      c->sig = (float)v;   // a sig array given earlier stays in use, as in py_update (:2596-2606)
       compute_totsig(c);
       break;
     case DASP_CENT_READNOISE: c->readnoise = (float)v; break;
@@ -1416,8 +1475,13 @@ int dasp_cent_update_array(dasp_cent_t *c, int code, const dasp_cent_array *va) 
       c->useBrightest = 0;
       c->haveUBArr = true;
       break;
-    case DASP_CENT_OUTPUTPHASEARR:
-      break;   // only meaningful with phase interpolation, which the device path rejects
+    case DASP_CENT_OUTPUTPHASEARR:   // (:2832-2848) the interpolated phase, for verification
+      c->h_outphase = nullptr;
+      if (!a.data) break;
+      if (a.ndim != 2 || a.dims[0] != c->npup || a.dims[1] != c->npup)
+        return fail(DASP_CENT_EINVAL, "outputPhaseArr should be 2D of size %d x %d", c->npup, c->npup);
+      c->h_outphase = (float *)a.data;
+      break;
     default: return fail(DASP_CENT_EINVAL, "centmodule: Code %d not recognised, nothing updated", code);
   }
   c->tablesDirty = true;

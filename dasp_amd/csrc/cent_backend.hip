@@ -466,10 +466,16 @@ __global__ __launch_bounds__(BT) void backend_kernel(BackArgs a) {
     }
     __syncthreads();
     if (a.calsource == 0) {
+      float thr = a.floor;
+      if (a.threshType == 2 && a.floor > 0.f && a.floor <= 1.f) {   // relative floor: f * max
+        float mx = -INFINITY;
+        for (int q = tid; q < nexposed; q += BT) mx = fmaxf(mx, img[q]);
+        thr = a.floor * fmaxf(block_max(mx, L), 0.f);
+      }
       for (int q = tid; q < nexposed; q += BT) {
         float v = img[q];
-        if (a.threshType == 0) v = v < a.floor ? 0.f : v - a.floor;
-        else if (a.threshType == 1) v = v < a.floor ? 0.f : v;
+        if (a.threshType == 0 || a.threshType == 2) v = v < thr ? 0.f : v - thr;
+        else if (a.threshType == 1) v = v < thr ? 0.f : v;
         img[q] = v;
       }
       __syncthreads();
@@ -638,8 +644,9 @@ __global__ __launch_bounds__(BT) void backend_kernel(BackArgs a) {
         sy += img[i + ncen];
         ay += img[i + ncen] * i;
       }
-      cx = sx > 0 ? ax / sx - (ncen - 1) / 2.f : 0.f;
-      cy = sy > 0 ? ay / sy - (ncen - 1) / 2.f : 0.f;
+      // computeOBCents (:852-872): with no positive flux the weighted sum itself is returned
+      cx = sx > 0 ? ax / sx - (ncen - 1) / 2.f : ax;
+      cy = sy > 0 ? ay / sy - (ncen - 1) / 2.f : ay;
     }
   } else if (a.doFit) {
     if (tid == 0)
@@ -705,12 +712,27 @@ __global__ void inactive_kernel(InactiveArgs a) {
     } else {
       float *b = a.bimg + (long)frame * a.bimg_stride + (long)s * M2;
       const float *z = a.zread ? a.zread + (long)frame * a.noise_stride + (long)s * M2 : nullptr;
+      const bool rn = a.readnoise != 0.f && z;
+      float thr = a.floor;
+      if (a.threshType == 2 && a.floor > 0.f && a.floor <= 1.f) {   // relative floor: f * max
+        __shared__ float red[BT];
+        float mx = -INFINITY;
+        for (int q = threadIdx.x; q < M2; q += blockDim.x) mx = fmaxf(mx, rn ? fmaf(a.readnoise, z[q], a.readbg) : a.readbg);
+        red[threadIdx.x] = mx;
+        __syncthreads();
+        for (int w = BT / 2; w > 0; w >>= 1) {
+          if ((int)threadIdx.x < w) red[threadIdx.x] = fmaxf(red[threadIdx.x], red[threadIdx.x + w]);
+          __syncthreads();
+        }
+        thr = a.floor * fmaxf(red[0], 0.f);
+        __syncthreads();
+      }
       for (int q = threadIdx.x; q < M2; q += blockDim.x) {
         float v = 0.f;
-        if (a.threshType == 0 || a.threshType == 1) {
-          v += (a.readnoise != 0.f && z) ? fmaf(a.readnoise, z[q], a.readbg) : a.readbg;
-          if (a.threshType == 0) v = v < a.floor ? 0.f : v - a.floor;
-          else v = v < a.floor ? 0.f : v;
+        if (a.threshType >= 0 && a.threshType <= 2) {   // other types: the reference leaves the data
+          v += rn ? fmaf(a.readnoise, z[q], a.readbg) : a.readbg;
+          if (a.threshType == 1) v = v < thr ? 0.f : v;
+          else v = v < thr ? 0.f : v - thr;
         }
         b[q] = v;
       }

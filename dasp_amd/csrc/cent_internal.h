@@ -259,12 +259,19 @@ struct BatchDesc {
   long group0;           // first queue entry of this WFS (groups of 64/NP work items)
 };
 
+// Kernel argument blocks travel by value between separately compiled objects: a layout change
+// must come with a full rebuild (dasp_amd/build.py compiles every source each time).
+static_assert(sizeof(FusedArgs) == 272, "FusedArgs layout changed: update this pin and rebuild all objects");
+static_assert(sizeof(BatchDesc) == 280, "BatchDesc layout changed: update this pin and rebuild all objects");
+
 struct BatchArgs {
   const BatchDesc *desc; // ascending group0, desc[0].group0 == 0
   int ndesc;
   long total_groups;
   unsigned long long *counters;   // [0] next queue entry, [1] waves finished; zero between launches
 };
+
+static_assert(sizeof(BatchArgs) == 32, "BatchArgs layout changed: update this pin and rebuild all objects");
 
 // Inactive subapertures of one WFS in a multi-WFS step: entries [start, start + nframes*ninactive).
 struct InactiveDesc {

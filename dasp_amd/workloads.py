@@ -30,6 +30,7 @@ class WFS:
     addPoisson: int = 0
     noiseFloor: float = 0.0
     threshType: int = 0
+    relativeFloor: bool = False   # engine option: threshType 0 floor = noiseFloor * image maximum
     correlation: int = 0     # correlationCentroiding
     parabolicFit: int = 0
     spot: bool = False       # per-subaperture LGS spot kernels
@@ -70,7 +71,7 @@ class WFS:
 # (LGS 80x80, NGS 8x8, truth 40x40) and the 80x80 Nyquist WFS with correlation.
 SCAO8 = WFS("scao8x8", 8, 8, 16, 8, 4.2, 0.25, 0.15, 30.0)
 NOISE40 = WFS("wfs40x40_noise", 40, 16, 32, 16, 8.0, 0.14, 0.12, 25.0, sig=1000.0, readnoise=0.5,
-         addPoisson=1, noiseFloor=0.1, threshType=2, nframes=64)
+         addPoisson=1, noiseFloor=0.1, threshType=0, relativeFloor=True, nframes=64)
 WFS80 = WFS("wfs80x80_nfft32", 80, 16, 32, 16, 39.0, 0.28, 0.157, 25.0, nframes=64)
 LGS80 = WFS("lgs80x80", 80, 16, 32, 16, 39.0, 0.28, 0.157, 25.0, sig=1000.0, readnoise=0.5,
              addPoisson=1, spot=True)

@@ -21,7 +21,8 @@
 extern "C" {
 #endif
 
-#define DASP_CENT_ABI_VERSION 1
+/* Bumped whenever a structure below changes layout; cent.py refuses a library of another version. */
+#define DASP_CENT_ABI_VERSION 2
 
 /* error codes */
 #define DASP_CENT_OK 0
@@ -157,6 +158,13 @@ int dasp_cent_testcorrelation(const float *img, float *corr, int n);
  * fftsize = clipsize = 32 and n = 17..32 / fftsize = clipsize = 64, the fused kernels' options and
  * no spot kernel; other geometries return DASP_CENT_EUNSUPPORTED. */
 int dasp_cent_set_form(dasp_cent_t *h, int form);
+
+/* Relative detector floor (no counterpart in the reference, whose readCCD knows threshType 0
+ * and 1 only and leaves the data untouched for any other type, cmod/src/centmodule.c:785-817):
+ * with enable != 0 and threshType 0, the floor subtracted and clipped at is noiseFloor * max(image)
+ * when 0 < noiseFloor <= 1 (thresholdCorrelation's rule, :1488-1505), noiseFloor itself otherwise.
+ * DASP_CENT_EINVAL when threshType is not 0. */
+int dasp_cent_set_relative_floor(dasp_cent_t *h, int enable);
 
 /* Phase factor exp(i(phi - t)) in the fused and DFT-GEMM kernels: 1 (default) = the argument
  * reduced to revolutions, r = x*c_hi - k + x*c_lo with c_hi + c_lo = 1/(2 pi) (reduction error at

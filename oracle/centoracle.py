@@ -8,37 +8,71 @@ Each function restates one piece of the reference engine ``cmod/src/centmodule.c
 Python helpers around it) in float64, citing the lines it follows. Citations are relative to the
 agb32/dasp source tree.
 
-How each function is pinned (tests/test_oracle_*.py). The reference holds no numeric golden
-vectors for this path, and it cannot be built here (FFTW3f, GSL, Python 2 C-API). Each function
-is therefore checked against mathematics or an independent route:
-  compute_hll            brute-force 2-D DFT on small inputs; Parseval (sum = F^2 * sum(pup));
-                         flat phase + symmetric pupil gives a centred spot (centroid 0);
-                         a ramp of one FFT pixel gives the closed-form Dirichlet-kernel centroid;
-                         transposing the phase swaps x and y.
-  add_power_spec_central independent index-by-index loop (tiny sizes), clip and pad cases.
-  prebin_image           block sums computed with reshape (textbook).
-  flip_array             equals numpy.fft.fftshift for even sizes (mirror=1); involution tests.
-  do_convolution         direct circular convolution (brute force) on small arrays; delta spot
-                         gives the identity times P^2.
-  bin_image_literal      integer ratio equals block sums; total flux preserved for every ratio;
-                         equals bin_matrix (the separable operator) on random inputs.
-  compute_cog            delta images give the pixel offset exactly; symmetric images give 0.
-  calc_correlation_hc    equals correlate_fft (numpy ifft2 route, util/correlation.py:177-186)
-                         and brute-force circular correlation.
-  transform_psf          its half-complex unpack equals conj(fft2(fftshift(psf))).
-  make_fit_mx/compute_fit exact vertex recovery for sampled paraboloids; fit matrix equals the
-                         inverse of the normal matrix from an independent construction.
-  gaussian_log           exact centre of a sampled 2-D Gaussian through the Gaussian fit; the
+How each function is pinned. The reference holds no numeric golden vectors for this path, and it
+cannot be built here (FFTW3f, GSL, Python 2 C-API). Each function is therefore checked against
+mathematics, the reference's C text read as closed forms, or an independent route; the checks
+live in tests/test_oracle_pins.py (P) and tests/test_oracle_pins_detector.py (D), where every
+pin of D also rejects deliberately planted mistakes (comparisons, dropped terms, off-by-one
+windows, shift directions) in a copy of the function:
+  compute_hll, tilt_fn,  (P) brute-force 2-D DFT on small inputs; Parseval (sum = F^2 * sum(pup));
+  subap_windows          flat phase + symmetric pupil gives a centred spot (centroid 0); a ramp
+                         of one FFT pixel gives the closed-form Dirichlet-kernel centroid;
+                         transposing the phase swaps x and y; wfsCent.py's orientation asserts.
+  add_power_spec_central (P) independent index-by-index loop (tiny sizes), clip and pad cases.
+  prebin_image           (P) block sums computed with reshape (textbook).
+  flip_array             (P) equals numpy.fft.fftshift for even sizes (mirror=1); involutions.
+  prepare_spot_otf,      (P) direct circular convolution (brute force) on small arrays; a delta
+  do_convolution         spot gives the identity times P^2.
+  tot_sig                (D) with a spot: images of a clip = P = F run sum to sig * fracSubArea
+                         (Parseval and the unnormalised r2c/c2r pair), 2-D and 4-D spots.
+  bin_weights,           (P) integer ratio equals block sums; total flux preserved for every
+  bin_image_literal,     ratio; equals bin_matrix (the separable operator) on random inputs.
+  bin_matrix
+  poisson_from_normal,   (P) exact inverse CDF below lambda=10 against pmf sums; mean/variance
+  normal_cdf             and total-variation distance to the exact pmf above it.
+  read_ccd               (D) floor ramps for types 0 and 1 (the pixel equal to the floor), read
+                         noise sigma*z + readBg, unknown types untouched, the relative floor
+                         option per image and its absolute and all-negative cases.
+  select_brightest       (D) permutations (top N kept, minus the next brightest for -N), ties at
+                         the threshold, N = 0 or >= npxl.
+  apply_power            (D) powers 2, 1, 0 (no power), 0.5 and 3.
+  optically_bin          (D) window column/row sums for ncen < nimg; for ncen = nimg the closed
+                         form of the reference's in-place overwrite of row 1.
+  ob_cents               (D) delta positions; a non-positive flux keeps the weighted sum.
+  r2hc*/hc2r*,           (P) calc_correlation_hc equals correlate_fft (numpy ifft2 route,
+  hc_multiply,           util/correlation.py:177-186) and brute-force circular correlation;
+  transform_psf,         the half-complex unpack equals conj(fft2(fftshift(psf))).
+  calc_correlation_hc,
+  correlate_fft,
+  hc_pattern_to_spectrum
+  threshold_correlation  (D) thresholds 0, 0.3 (fraction of the maximum), 1 and 2 (absolute) on a
+                         signed ramp; an all-negative image.
+  diff_correlation       (P, D) modes 1 and 3: a pattern moved by a known shift gives its only
+                         zero at that shift, constant images give the closed-form normalisation;
+                         the fallback loop (correlationCentroiding 3): the same with its
+                         opposite shift convention, and (n-|dy|)(n-|dx|) for constant images.
+  brute_correlation      (P, D) zero shift is the sum of squares; two deltas correlate only at
+                         the shift that brings them together.
+  compute_cog            (P, D) delta images give the pixel offset exactly; symmetric images give
+                         0; weighted: two deltas with weights, the ncen window, zero weighted flux.
+  make_fit_mx,           (P) exact vertex recovery for sampled paraboloids; fit matrix equals the
+  compute_fit            inverse of the normal matrix from an independent construction.
+  gaussian_log           (P) exact centre of a sampled 2-D Gaussian through the Gaussian fit; the
                          returned image of a Gaussian-fit run equals the log of a plain run's.
-  poisson_from_normal    exact inverse-CDF below lambda=10 (checked against scipy-free pmf sums);
-                         mean/variance and total-variation distance to the exact pmf above it.
-  diff_correlation, brute_correlation
-                         brute correlation equals a direct shifted-product loop; zero-shift of
-                         identical images gives the minimum of the difference measures.
-  interpolate_phase      reproduces linear and quadratic surfaces exactly away from the edges
-                         (a property of the Keys cubic kernel the reference uses).
-Functions without such a check: none.
+  interp1                (D) knots, midpoints and clamping of a piecewise-linear table.
+  apply_cal_poly         (D) polynomial identities ((1+c)^2, c^3, constants, empty).
+  interpolate_phase      (P) reproduces linear and quadratic surfaces exactly away from the edges
+                         (a property of the Keys cubic kernel the reference uses); edge samples.
+  run_image              (D) subapLocation: the default tiling equals no locations; a smaller
+                         window lands in the image's top-left corner, keeps the previous pixels
+                         elsewhere and floors only the exposed ones.
+  run_phase, _image_backend
+                         compositions of the functions above; pinned through them and through
+                         the whole-pipeline identities (flat phase, Parseval totals, ramps).
+Functions without an independent check: none of the above; the exact FFTW/GSL arithmetic and
+noise streams of the reference are outside the oracle (pre-drawn normals replace them).
 """
+import functools
 import math
 
 import numpy as np
@@ -329,30 +363,30 @@ def poisson_from_normal(lam, z):
     return out
 
 
-def read_ccd(img, zread, readbg, readnoise, floor, thresh_type):
+def read_ccd(img, zread, readbg, readnoise, floor, thresh_type, relative_floor=False):
     """readCCD, cmod/src/centmodule.c:785-817 with the Gaussian draw replaced by readnoise*zread
     (zread pre-drawn N(0,1)); readnoise==0 adds readbg only. thresh_type 0: x<f?0:x-f;
-    1: x<f?0:x; 2 (extension, not in the reference): the floor is f*max(x) when 0<f<=1
-    (thresholdCorrelation semantics, :1488-1505), then subtract-and-clip."""
+    1: x<f?0:x; any other type: the data are left untouched (no noise, no readbg, no floor).
+    relative_floor (engine option, not in the reference; thresh_type 0 only): the floor of each
+    image (first axis) is f*max(x) when 0<f<=1 (thresholdCorrelation's rule, :1488-1505)."""
     x = np.asarray(img, f64)
-    if thresh_type not in (0, 1, 2):
+    if thresh_type not in (0, 1):
         return x   # the reference prints "Unknown threshold type" and leaves the data (:797, :813)
     if readnoise == 0 or zread is None:
         x = x + readbg
     else:
         x = x + (readnoise * np.asarray(zread, f64) + readbg)
-    if thresh_type == 0:
-        return np.where(x < floor, 0.0, x - floor)
     if thresh_type == 1:
         return np.where(x < floor, 0.0, x)
-    if thresh_type == 2:
-        S = x.shape[0]
-        if 0 < floor <= 1:
-            thr = floor * np.maximum(x.reshape(S, -1).max(axis=1), 0.0)
-        else:
-            thr = np.full(S, floor)
-        thr = thr.reshape((S,) + (1,) * (x.ndim - 1))
-        return np.where(x < thr, 0.0, x - thr)
+    if not relative_floor:
+        return np.where(x < floor, 0.0, x - floor)
+    S = x.shape[0]
+    if 0 < floor <= 1:
+        thr = floor * np.maximum(x.reshape(S, -1).max(axis=1), 0.0)
+    else:
+        thr = np.full(S, float(floor))
+    thr = thr.reshape((S,) + (1,) * (x.ndim - 1))
+    return np.where(x < thr, 0.0, x - thr)
 
 
 def select_brightest(img, use_brightest):
@@ -409,7 +443,8 @@ def optically_bin(img, nimg, ncen):
 
 
 def ob_cents(img, ncen):
-    """computeOBCents, cmod/src/centmodule.c:852-872."""
+    """computeOBCents, cmod/src/centmodule.c:852-872. Without positive flux in a direction the
+    reference leaves that direction's weighted sum in place (it is not reset to 0)."""
     a = np.asarray(img, f64).ravel()
     i = np.arange(ncen)
     sx = a[:ncen].sum()
@@ -418,12 +453,8 @@ def ob_cents(img, ncen):
     cy = (a[ncen:2 * ncen] * i).sum()
     if sx > 0:
         cx = cx / sx - (ncen - 1) / 2.0
-    else:
-        cx = 0.0
     if sy > 0:
         cy = cy / sy - (ncen - 1) / 2.0
-    else:
-        cy = 0.0
     return cx, cy
 
 
@@ -726,7 +757,19 @@ def compute_cog(img, nimg, ncen, weighting=None, centre_on_brightest=0):
     return 0.0, 0.0
 
 
+@functools.lru_cache(maxsize=None)
+def _fit_mx_cached(fitsize):
+    r = make_fit_mx_uncached(fitsize)
+    r.setflags(write=False)
+    return r
+
+
 def make_fit_mx(fitsize):
+    """makeFitMx (see make_fit_mx_uncached), computed once per fit size."""
+    return _fit_mx_cached(int(fitsize))
+
+
+def make_fit_mx_uncached(fitsize):
     """makeFitMx, cmod/src/centmodule.c:997-1057: first 5 rows of the inverse of the 6x6 normal
     matrix of the basis [x^2, xy, y^2, x, y, 1] over a fitsize x fitsize grid."""
     y, x = np.mgrid[0:fitsize, 0:fitsize].astype(f64)
@@ -919,7 +962,7 @@ class Params:
                  corrThresh=0.0, corrPattern=None, threshType=0, imageOnly=0, useBrightest=0,
                  useBrightestArr=None, preBinningFactor=1, parabolicFit=0, gaussianFit=0,
                  gaussianMinVal=0.0, gaussianReplaceVal=0.0, refCents=None, calData=None,
-                 calBounds=None, calSteps=None, calCoeff=None)
+                 calBounds=None, calSteps=None, calCoeff=None, relativeFloor=False)
         for k, v in kw.items():
             if k not in d:
                 raise TypeError("unknown parameter %s" % k)
@@ -939,6 +982,22 @@ class Params:
         if self.correlationCentroiding and self.corrPattern is not None:
             return np.asarray(self.corrPattern).shape[-1]
         return self.nimg
+
+
+def _as_f64(p):
+    """A copy of the parameters with the per-subaperture arrays the backend reads converted to
+    float64 once (the backend indexes them per subaperture)."""
+    q = Params.__new__(Params)
+    q.__dict__.update(p.__dict__)
+    for name in ("corrPattern", "centWeight", "calData", "calSteps", "calCoeff", "refCents"):
+        v = getattr(p, name)
+        if v is not None:
+            setattr(q, name, np.asarray(v, f64))
+    for name in ("useBrightestArr", "calBounds"):
+        v = getattr(p, name)
+        if v is not None:
+            setattr(q, name, np.asarray(v))
+    return q
 
 
 def _image_backend(p, img, s, nexposed_full):
@@ -1058,6 +1117,7 @@ def run_phase(p, phs, zpois=None, zread=None, bimg_in=None):
     if p.skybrightnessArr is not None:
         img = img + np.asarray(p.skybrightnessArr, f64).reshape(S, nimg, nimg)
     cents = np.zeros((S, 2), f64)
+    p = _as_f64(p)
     bimg = np.zeros((S, nimg, nimg), f64) if bimg_in is None else np.array(bimg_in, f64).reshape(S, nimg, nimg)
     cs = p.corrsize()
     corrimg = None
@@ -1069,7 +1129,7 @@ def run_phase(p, phs, zpois=None, zread=None, bimg_in=None):
                 rn = p.readnoise * (p.calsource == 0)
                 z = None if zread is None else zread[s]
                 bimg[s] = read_ccd(np.zeros((1, nimg, nimg)), None if z is None else z[None],
-                                   p.readbg, rn, p.noiseFloor, p.threshType)[0]
+                                   p.readbg, rn, p.noiseFloor, p.threshType, p.relativeFloor)[0]
             continue
         x = img[s]
         if p.opticalBinning == 1:
@@ -1085,7 +1145,8 @@ def run_phase(p, phs, zpois=None, zread=None, bimg_in=None):
         rn = p.readnoise * (p.calsource == 0)
         flat = x.ravel().copy()
         zr = None if zread is None else np.asarray(zread[s], f64).ravel()[:nexp][None]
-        flat[:nexp] = read_ccd(flat[:nexp][None], zr, p.readbg, rn, p.noiseFloor, p.threshType)[0]
+        flat[:nexp] = read_ccd(flat[:nexp][None], zr, p.readbg, rn, p.noiseFloor, p.threshType,
+                               p.relativeFloor)[0]
         x = flat.reshape(nimg, nimg)
         out_img, c, corr = _image_backend(p, x, s, nexp)
         bimg[s] = out_img
@@ -1102,6 +1163,7 @@ def run_image(p, image, bimg_in=None, subap_location=None, zread=None):
     S = p.nsubaps
     nimg = p.nimg
     image = np.asarray(image, f64)
+    p = _as_f64(p)
     cents = np.zeros((S, 2), f64)
     bimg = np.zeros((S, nimg, nimg), f64) if bimg_in is None else np.array(bimg_in, f64).reshape(S, nimg, nimg)
     cs = p.corrsize()
@@ -1114,7 +1176,7 @@ def run_image(p, image, bimg_in=None, subap_location=None, zread=None):
                 rn = p.readnoise * (p.calsource == 0)
                 z = None if zread is None else np.asarray(zread[s], f64)[None]
                 bimg[s] = read_ccd(np.zeros((1, nimg, nimg)), z, p.readbg, rn,
-                                   p.noiseFloor, p.threshType)[0]
+                                   p.noiseFloor, p.threshType, p.relativeFloor)[0]
             continue
         x = bimg[s].copy()
         if image.ndim == 2:
@@ -1136,7 +1198,8 @@ def run_image(p, image, bimg_in=None, subap_location=None, zread=None):
             nexp = nimg * nimg
         if p.calsource == 0:
             flat = x.ravel().copy()
-            flat[:nexp] = read_ccd(flat[:nexp][None], None, 0.0, 0.0, p.noiseFloor, p.threshType)[0]
+            flat[:nexp] = read_ccd(flat[:nexp][None], None, 0.0, 0.0, p.noiseFloor, p.threshType,
+                                   p.relativeFloor)[0]
             x = flat.reshape(nimg, nimg)
         out_img, c, corr = _image_backend(p, x, s, nexp)
         bimg[s] = out_img

@@ -15,7 +15,7 @@ class CrefParams(ctypes.Structure):
                 ("pupfn", ctypes.c_void_p), ("subflag", ctypes.c_void_p), ("fracSubArea", ctypes.c_void_p),
                 ("sig", ctypes.c_float), ("sigArr", ctypes.c_void_p), ("skybrightness", ctypes.c_float),
                 ("addPoisson", ctypes.c_int), ("calsource", ctypes.c_int), ("readnoise", ctypes.c_float),
-                ("readbg", ctypes.c_float), ("noiseFloor", ctypes.c_float), ("threshType", ctypes.c_int),
+                ("readbg", ctypes.c_float), ("noiseFloor", ctypes.c_float), ("threshType", ctypes.c_int), ("relativeFloor", ctypes.c_int),
                 ("pxlPower", ctypes.c_float), ("spotpsf", ctypes.c_void_p), ("psfsize", ctypes.c_int),
                 ("correlation", ctypes.c_int), ("corrPattern", ctypes.c_void_p), ("corrsize", ctypes.c_int),
                 ("corrThresh", ctypes.c_float), ("parabolicFit", ctypes.c_int), ("refCents", ctypes.c_void_p)]
@@ -46,7 +46,7 @@ class CRef:
     def __init__(self, nsubx, n, fftsize, clipsize, nimg, ncen, pupfn, subflag, frac, sig=1.0, sigArr=None,
                  skybrightness=0., addPoisson=0, calsource=0, readnoise=0., readbg=0., noiseFloor=0.,
                  threshType=0, pxlPower=1., spotpsf=None, corrPattern=None, corrThresh=0., parabolicFit=0,
-                 refCents=None):
+                 refCents=None, relativeFloor=False):
         f32 = lambda a: None if a is None else np.ascontiguousarray(a, np.float32)
         self.keep = dict(pupfn=f32(pupfn), subflag=np.ascontiguousarray(subflag, np.int32), frac=f32(frac),
                          sigArr=f32(sigArr), spot=f32(spotpsf), corr=f32(corrPattern), ref=f32(refCents))
@@ -56,7 +56,7 @@ class CRef:
         self.C = corrPattern.shape[-1] if corrPattern is not None else nimg
         self.p = CrefParams(nsubx, n, fftsize, clipsize, nimg, ncen, _p(k["pupfn"]), _p(k["subflag"]),
                             _p(k["frac"]), sig, _p(k["sigArr"]), skybrightness, addPoisson, calsource, readnoise,
-                            readbg, noiseFloor, threshType, pxlPower, _p(k["spot"]),
+                            readbg, noiseFloor, threshType, int(bool(relativeFloor)), pxlPower, _p(k["spot"]),
                             spotpsf.shape[-1] if spotpsf is not None else 0, 1 if corrPattern is not None else 0,
                             _p(k["corr"]), self.C, corrThresh, parabolicFit, _p(k["ref"]))
 

@@ -10,7 +10,7 @@
  * (:524-564, complex transforms instead of r2c/c2r), binImage with the prepareBinImage weights
  * (:565-651), the theoretical-total normalisation (:1829-1863, :2403-2427), sky, Poisson and read
  * noise from pre-drawn N(0,1) arrays (the same transform as the GPU engine), readCCD floors
- * (:785-817, plus the relative floor threshType 2), pixel power, the FFT correlation with its
+ * (:785-817, plus the engine's relative floor option), pixel power, the FFT correlation with its
  * half-complex product (:1403-1505), computeCoG (:873-971) and computeFit (:1069-1188).
  * Work is split over a pthread pool with the reference's static contiguous partition
  * (:2533-2550).
@@ -31,6 +31,7 @@ typedef struct {
   int addPoisson, calsource;
   float readnoise, readbg, noiseFloor;
   int threshType;
+  int relativeFloor;       /* engine option on top of threshType 0: floor = noiseFloor * max */
   float pxlPower;
   const float *spotpsf;    /* NULL or (S, P, P) */
   int psfsize;
@@ -408,12 +409,12 @@ static void *worker(void *arg) {
     }
     {
       float rn = p->calsource == 0 ? p->readnoise : 0.f, thr = p->noiseFloor, mx = -INFINITY;
-      if (p->threshType >= 0 && p->threshType <= 2) {
+      if (p->threshType == 0 || p->threshType == 1) {   /* other types: readCCD leaves the data */
         for (i = 0; i < M2; ++i) {
           img[i] += (rn != 0.f && jb->zread) ? rn * jb->zread[(long)s * M2 + i] + p->readbg : p->readbg;
           if (img[i] > mx) mx = img[i];
         }
-        if (p->threshType == 2 && thr > 0.f && thr <= 1.f) thr = thr * (mx > 0 ? mx : 0);
+        if (p->threshType == 0 && p->relativeFloor && thr > 0.f && thr <= 1.f) thr = thr * (mx > 0 ? mx : 0);
         for (i = 0; i < M2; ++i) {
           if (p->threshType == 1) img[i] = img[i] < thr ? 0.f : img[i];
           else img[i] = img[i] < thr ? 0.f : img[i] - thr;

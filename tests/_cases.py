@@ -21,7 +21,7 @@ class Case:
                  nintegrations=1, nlatency=0, seed=1, spotpsf=None, opticalBinning=0, centWeight=None,
                  correlationCentroiding=0, corrThresh=0., corrPattern=None, threshType=0, imageOnly=0,
                  useBrightest=0, preBinningFactor=1, parabolicFit=0, gaussianFitVals=None,
-                 inputImage=None, subapLocation=None, phs=None, pupil=None, form=None):
+                 inputImage=None, subapLocation=None, phs=None, pupil=None, form=None, relativeFloor=False):
         self.nsubx, self.n, self.nfft, self.nimg = nsubx, n, nfft, nimg
         self.clip = clip or nfft
         self.ncen = ncen or nimg
@@ -42,6 +42,9 @@ class Case:
             self.cents, self.bimg, 0.5, opticalBinning, centWeight, correlationCentroiding, corrThresh, hc,
             self.corrimg, threshType, imageOnly, useBrightest, preBinningFactor, parabolicFit, gaussianFitVals,
             inputImage, subapLocation)
+        if relativeFloor:   # engine option: threshType 0 with the floor relative to the image maximum
+            from dasp_amd.cmod import cent
+            cent.setRelativeFloor(self.cc.centstruct, True)
         if form is not None:
             from dasp_amd.cmod import cent
             cent.setForm(self.cc.centstruct, form % 10)
@@ -61,7 +64,7 @@ class Case:
             useBrightest=0 if ubarr is not None else int(useBrightest), useBrightestArr=ubarr,
             preBinningFactor=preBinningFactor, parabolicFit=parabolicFit, gaussianFit=gf,
             gaussianMinVal=0. if gf == 0 else gaussianFitVals[0],
-            gaussianReplaceVal=0. if gf == 0 else gaussianFitVals[1])
+            gaussianReplaceVal=0. if gf == 0 else gaussianFitVals[1], relativeFloor=relativeFloor)
         self.seed = seed
         self.readouts = 0
 

@@ -1,8 +1,12 @@
-"""C-ABI checks that need no GPU: the library loads, exports every symbol include/dasp_cent.h
-declares, its host-only helpers agree with the oracle, and GPU entry points fail loudly here."""
+"""C-ABI checks that need no GPU: the C library and the cent extension load, the library exports
+every symbol include/dasp_cent.h declares, the extension imports under the reference's name
+cmod.cent, its host-only helpers agree with the oracle, and GPU entry points fail loudly here."""
 import ctypes
 import os
 import re
+import subprocess
+import sys
+import sysconfig
 
 import numpy as np
 import pytest
@@ -14,17 +18,45 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 def _lib():
     from dasp_amd.cmod import cent
-    return cent, cent.library()
+    return cent, ctypes.CDLL(os.path.join(ROOT, "dasp_amd", "libdasp_cent.so"))
 
 
 def test_exports_every_declared_symbol():
-    _, lib = _lib()
+    """Both the C library and the extension module carry every entry point of the header."""
+    cent, lib = _lib()
     hdr = open(os.path.join(ROOT, "include", "dasp_cent.h")).read()
     names = sorted(set(re.findall(r"\b(dasp_cent_\w+)\s*\(", hdr)))
     assert len(names) >= 16
+    ext = ctypes.CDLL(cent.__file__)
     for nm in names:
         assert hasattr(lib, nm), nm
-    assert lib.dasp_cent_abi_version() == 1
+        assert hasattr(ext, nm), nm
+    assert lib.dasp_cent_abi_version() == 2 == cent.abiVersion() == cent.ABI_VERSION
+
+
+def test_extension_imports_as_cmod_cent(tmp_path):
+    """The built extension is the module DASP imports as cmod.cent (util/centcmod.py:16): dropped
+    into a cmod package under its own file name it imports with no Python shim, has the six
+    reference functions and cent.error, and runs a host-only function."""
+    from dasp_amd.cmod import cent
+    pkg = tmp_path / "cmod"
+    pkg.mkdir()
+    (pkg / "__init__.py").write_text("")
+    os.symlink(cent.__file__, pkg / ("cent" + sysconfig.get_config_var("EXT_SUFFIX")))
+    code = ("import numpy as np, cmod.cent as c\n"
+            "assert all(hasattr(c, f) for f in ('initialise', 'run', 'runSlope', 'update', 'free', 'testcorrelation'))\n"
+            "assert issubclass(c.error, Exception) and c.abiVersion() == 2\n"
+            "img = np.zeros((8, 8), np.float32); img[3, 4] = 1\n"
+            "corr = np.zeros((8, 8), np.float32); c.testcorrelation(img, corr)\n"
+            "assert not corr.any()\n"
+            "try:\n"
+            "    c.run(12345)\n"
+            "except c.error as e:\n"
+            "    print('ok', e)\n")
+    r = subprocess.run([sys.executable, "-c", code], cwd=str(tmp_path), capture_output=True, text=True,
+                       env=dict(os.environ, PYTHONPATH=str(tmp_path)))
+    assert r.returncode == 0, r.stderr[-2000:]
+    assert r.stdout.startswith("ok invalid centstruct handle"), r.stdout
 
 
 def test_noise_stream_is_deterministic_normal():

@@ -33,7 +33,7 @@ def test_cref_noise_same_normals():
     rng = np.random.default_rng(4)
     zp = rng.standard_normal((S, 16, 16)).astype(np.float32)
     zr = rng.standard_normal((S, 16, 16)).astype(np.float32)
-    kw = dict(sig=1000.0, addPoisson=1, readnoise=0.5, noiseFloor=0.1, threshType=2)
+    kw = dict(sig=1000.0, addPoisson=1, readnoise=0.5, noiseFloor=0.1, threshType=0, relativeFloor=True)
     ph = w.phase()
     c, b, _ = CRef(w.nsubx, w.n, w.nfft, w.clip, w.nimg, w.cen, pupfn, fl, fr, **kw).run(ph, zp, zr)
     oc, ob, _ = O.run_phase(O.Params(w.nsubx, w.n, w.nfft, w.clip, w.nimg, w.cen, pupfn, fl, fr, **kw), ph, zp, zr)

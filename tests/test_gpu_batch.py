@@ -17,7 +17,7 @@ MIX = [
     ("scao8x8_n8", 8, 8, 16, 8, {}),
     ("wfs12_n16", 12, 16, 32, 16, {}),
     ("ngs4_n32_b8", 4, 32, 64, 8, {}),
-    ("wfs10_noise", 10, 16, 32, 16, dict(sig=1000.0, readnoise=0.5, addPoisson=1, noiseFloor=0.1, threshType=2)),
+    ("wfs10_noise", 10, 16, 32, 16, dict(sig=1000.0, readnoise=0.5, addPoisson=1, noiseFloor=0.1, threshType=0, relativeFloor=True)),
     ("wfs6_n16_b4", 6, 16, 32, 8, {}),
 ]
 
@@ -201,6 +201,63 @@ def test_batch_with_dft_form_members():
             d = np.abs(got - want).max()
             say("replay %d member %d: max|batch - per-WFS| = %.3g px" % (rep, k, d))
             assert d <= 1e-5
+    b.free()
+    for c in cases:
+        c.free()
+
+
+def test_batch_rebuilds_after_member_scratch_grows():
+    """A member that runs its own kernels in the step graph (n32 DFT form with the correlation
+    stage, no image buffer given) uses the handle's image scratch. A device run of that handle
+    with more frames moves the scratch; the next step must rebuild its graph on the new address
+    (the old one is freed) and still equal the eager per-WFS run, on two steps in a row."""
+    import torch
+    from dasp_amd.cmod import cent
+    from dasp_amd.wfsbatch import WFSBatch
+    say("case batch member scratch grows between steps")
+    w32 = W.WFS("d32", 8, 32, 64, 16, 8.0, 0.2, 0.15, 25.0)
+    cases = [Case(8, 32, 64, 16, obs=0.2, form=11, correlationCentroiding=1,
+                  corrPattern=W.reference_images(w32), parabolicFit=5),
+             Case(12, 16, 32, 16, obs=0.2, form=10)]
+    frames = 2
+    b = WFSBatch([c.cc for c in cases], frames)
+    wgeo = [w32, W.WFS("f16", 12, 16, 32, 16, 8.0, 0.2, 0.15, 25.0)]
+    for k in range(len(cases)):
+        for f in range(frames):
+            b.phs[k][f].copy_(torch.from_numpy(wgeo[k].phase(f)))
+    torch.cuda.synchronize()
+    b.run()
+    torch.cuda.synchronize()
+    st = torch.cuda.current_stream().cuda_stream
+    c0 = cases[0]
+    npup = c0.nsubx * c0.n
+    big = 5   # more frames than the batch: the handle's image scratch is reallocated
+    ph = torch.zeros((big, npup, npup), dtype=torch.float32, device="cuda:0")
+    ph[:frames].copy_(b.phs[0])
+    ph[frames:].copy_(b.phs[0][:1])
+    ce_big = torch.zeros((big, c0.nsubx * c0.nsubx, 2), dtype=torch.float32, device="cuda:0")
+    cent.runDevice(c0.cc.centstruct, ph.data_ptr(), npup * npup, big, ce_big.data_ptr(), stream=st)
+    torch.cuda.synchronize()
+    for rep in range(2):
+        for ce in b.cents:
+            ce.fill_(-7.0)
+        b.run()
+        torch.cuda.synchronize()
+        for k, c in enumerate(cases):
+            ce = torch.full_like(b.cents[k], -9.0)
+            n2 = (c.nsubx * c.n) ** 2
+            nact, _, csz = cent.info(c.cc.centstruct)
+            bi = torch.zeros((frames, c.nsubx * c.nsubx, c.nimg, c.nimg), dtype=torch.float32, device="cuda:0")
+            ci = torch.zeros((frames, c.nsubx * c.nsubx, csz, csz), dtype=torch.float32, device="cuda:0")
+            cent.runDevice(c.cc.centstruct, b.phs[k].data_ptr(), n2, frames, ce.data_ptr(), bi.data_ptr(),
+                           ci.data_ptr(), stream=st)
+            torch.cuda.synchronize()
+            d = np.abs(b.slopes(k) - ce.cpu().numpy()).max()
+            say("step %d member %d after scratch growth: max|batch - per-WFS| = %.3g px" % (rep, k, d))
+            assert d <= 1e-5
+    # the big run's first frames equal the batch's frames
+    d = np.abs(ce_big[:frames].cpu().numpy().reshape(frames, -1) - b.slopes(0).reshape(frames, -1)).max()
+    assert d <= 1e-5
     b.free()
     for c in cases:
         c.free()

@@ -63,7 +63,7 @@ def test_noise_poisson_read_relative_floor(path):
     w = W.NOISE40
     say("case noise %s %s" % (w.name, path))
     c = Case(w.nsubx, w.n, w.nfft, w.nimg, obs=w.obs, sig=w.sig, readnoise=w.readnoise, addPoisson=1,
-             noiseFloor=w.noiseFloor, threshType=2, phs=w.phase(), seed=5, form=PATHS[path])
+             noiseFloor=w.noiseFloor, threshType=0, relativeFloor=True, phs=w.phase(), seed=5, form=PATHS[path])
     oc, ob, _ = c.run_both()
     act = _active(c)
     b = c.bimg.reshape(-1, w.nimg, w.nimg)[act]
@@ -102,7 +102,7 @@ LGS_CASES = {
     "n16_p32_4d": (10, 16, 32, 16, 32, 4, {}),              # fused (psfsize == clipsize == fftsize)
     "n16_p32_2d": (10, 16, 32, 16, 32, 2, {}),
     "n8_p16_4d": (8, 8, 16, 8, 16, 4, {}),
-    "n16_p32_noise": (10, 16, 32, 16, 32, 4, dict(readnoise=0.5, addPoisson=1, noiseFloor=0.05, threshType=2)),
+    "n16_p32_noise": (10, 16, 32, 16, 32, 4, dict(readnoise=0.5, addPoisson=1, noiseFloor=0.05, threshType=0, relativeFloor=True)),
     "n16_p64_4d": (6, 16, 32, 16, 64, 4, {}),              # psfsize > fftsize: general path only
 }
 
@@ -290,7 +290,7 @@ DFT16_CASES = {
     "floor_type0": dict(noiseFloor=0.004, threshType=0),
     "floor_type1": dict(noiseFloor=0.004, threshType=1),
     "ncen12_sky": dict(ncen=12, skybrightness=0.001),
-    "noise_relfloor": dict(sig=1000.0, readnoise=0.5, addPoisson=1, noiseFloor=0.1, threshType=2, seed=4),
+    "noise_relfloor": dict(sig=1000.0, readnoise=0.5, addPoisson=1, noiseFloor=0.1, threshType=0, relativeFloor=True, seed=4),
 }
 
 
@@ -359,7 +359,7 @@ def test_dft32_mfma_form_error_vs_fp64(name, form):
     if corr:
         kw.update(correlationCentroiding=1, corrPattern=W.reference_images(wgeo), parabolicFit=5)
     if noise:
-        kw.update(sig=1000.0, readnoise=0.5, addPoisson=1, noiseFloor=0.1, threshType=2, seed=9)
+        kw.update(sig=1000.0, readnoise=0.5, addPoisson=1, noiseFloor=0.1, threshType=0, relativeFloor=True, seed=9)
     c = Case(nsubx, n, 64, 16, **kw)
     oc, ob, _ = c.run_both()
     act = _active(c)
