@@ -35,6 +35,7 @@ static_assert(sizeof(dasp_cent_params) == 856, "dasp_cent_params layout changed"
 static_assert(offsetof(dasp_cent_params, seed) == 168 && offsetof(dasp_cent_params, phs) == 176 &&
               offsetof(dasp_cent_params, subapLocation) == 808, "dasp_cent_params field offsets changed");
 static_assert(sizeof(dasp_cent_batch_item) == 72, "dasp_cent_batch_item layout changed");
+static_assert(DASP_CENT_NOALLOC_SCRATCH_BYTES == (long)kNoAllocScratchBytes, "fixed scratch size out of step");
 
 namespace {
 
@@ -56,11 +57,20 @@ int fail(int code, const char *fmt, ...) {
     if (_e != hipSuccess) return fail(DASP_CENT_EHIP, "%s: %s", #expr, hipGetErrorString(_e)); \
   } while (0)
 
+// Set while dasp_cent_run_device_noalloc runs on this thread: a device buffer that would have to
+// grow fails instead of calling hipFree/hipMalloc.
+thread_local bool t_forbid_alloc = false;
+thread_local bool t_alloc_refused = false;
+
 struct DevBuf {
   void *p = nullptr;
   size_t bytes = 0;
   hipError_t ensure(size_t nb) {
     if (nb <= bytes && p) return hipSuccess;
+    if (t_forbid_alloc && nb > 0) {
+      t_alloc_refused = true;
+      return hipErrorOutOfMemory;
+    }
     if (p) (void)hipFree(p);
     p = nullptr;
     bytes = 0;
@@ -124,6 +134,7 @@ struct dasp_cent {
   bool tablesDirty = true;
   bool relFloor = false;   // engine option: threshType 0 floor taken relative to the image maximum
   unsigned long scratchEpoch = 0;   // bumped whenever a scratch buffer of the handle moves
+  void *noallocScratch = nullptr;   // the fixed scratch array on this handle's device
   std::vector<int> active, inactive, subflag;
   std::vector<float> pupfn, frac, sigArr, tilt, totsig, spot;
   int fitMxSize = -999;
@@ -548,25 +559,32 @@ static void fused_args(dasp_cent *c, const float *d_phs, long phs_stride, int nf
   fa.otf_per_subap = c->spotDim == 4;
 }
 
-// Phase -> centroids for nframes frames already resident on the device.
+// Device scratch of one phase -> slopes call: images and correlation images the kernels need but
+// the caller did not provide, the resampled phase grid and the general path's hll.
+struct Scratch {
+  float *bimg = nullptr, *corr = nullptr, *interp = nullptr, *hll = nullptr;
+};
+
+// Phase -> centroids for nframes frames already resident on the device. d_bimg / d_corrimg are
+// the images the caller gets back (or the scratch standing in for them).
 static int launch_phase(dasp_cent *c, const float *d_phs, long phs_stride, int nframes, float *d_cents,
                         float *d_bimg, float *d_corrimg, const float *d_zpois, const float *d_zread,
-                        hipStream_t st) {
+                        const Scratch &sc, hipStream_t st) {
   int rc = upload_tables(c);
   if (rc) return rc;
   if (c->interpolate) {   // interpolatePhase (:1628-1758) onto the nsubx*phasesize grid
     const long tot2 = (long)c->npup * c->npup;
-    HIPCHK(grow_scratch(c, c->d_interp, (size_t)nframes * tot2 * 4));
+    if (!sc.interp) return fail(DASP_CENT_EINVAL, "internal: no interpolation scratch");
     InterpArgs ia{};
     ia.in = d_phs;
     ia.in_stride = phs_stride;
     ia.m = c->inputPhaseSize;
-    ia.out = c->d_interp.as<float>();
+    ia.out = sc.interp;
     ia.tot = c->npup;
     ia.scale = (float)((c->inputPhaseSize - 1.) / (c->npup - 1));
     ia.nframes = nframes;
     HIPCHK(launch_interp(ia, st));
-    d_phs = c->d_interp.as<float>();
+    d_phs = sc.interp;
     phs_stride = tot2;
   }
   const long nitems = (long)nframes * (long)c->active.size();
@@ -621,8 +639,8 @@ static int launch_phase(dasp_cent *c, const float *d_phs, long phs_stride, int n
     }
   } else if (nitems > 0) {
     const long hll_elems = nitems * (long)c->padded * c->padded;
-    HIPCHK(grow_scratch(c, c->d_hll, hll_elems * sizeof(float)));
-    if (c->padded > c->F) HIPCHK(launch_fill(c->d_hll.as<float>(), hll_elems, 0.f, st));
+    if (!sc.hll) return fail(DASP_CENT_EINVAL, "internal: no image-formation scratch");
+    if (c->padded > c->F) HIPCHK(launch_fill(sc.hll, hll_elems, 0.f, st));
     FormArgs fa{};
     fa.phs = d_phs;
     fa.amp = amp;
@@ -636,7 +654,7 @@ static int launch_phase(dasp_cent *c, const float *d_phs, long phs_stride, int n
     fa.n = c->n;
     fa.npup = c->npup;
     fa.padded = c->padded;
-    fa.hll = c->d_hll.as<float>();
+    fa.hll = sc.hll;
     int trc = timing_mark(c, st);
     if (trc) return trc;
     HIPCHK(launch_form(fa, np, c->F, st));
@@ -644,7 +662,7 @@ static int launch_phase(dasp_cent *c, const float *d_phs, long phs_stride, int n
     if (trc) return trc;
     BackArgs b;
     fill_back(c, b, nframes);
-    b.hll = c->d_hll.as<float>();
+    b.hll = sc.hll;
     b.amp = amp;
     b.phs_stride = phs_stride;
     b.zpois = d_zpois;
@@ -750,21 +768,60 @@ static int check_device_run(dasp_cent *c, const float *d_phs, long phs_stride, i
   return 0;
 }
 
-// Images and correlation images the general path materialises even when the caller wants none.
-static int device_scratch(dasp_cent *c, int nframes, float **bimg, float **corr) {
-  if (!*bimg && (!use_fused(c) || corr_stage(c))) {
-    HIPCHK(grow_scratch(c, c->d_bscratch, (size_t)nframes * c->nsubaps * c->nimg * c->nimg * 4));
-    *bimg = c->d_bscratch.as<float>();
-  }
-  if (!*corr && c->correlationCentroiding && c->haveCorr) {
-    HIPCHK(grow_scratch(c, c->d_cscratch, (size_t)nframes * c->nsubaps * c->corrsize * c->corrsize * 4));
-    *corr = c->d_cscratch.as<float>();
-  }
-  if (c->interpolate) HIPCHK(grow_scratch(c, c->d_interp, (size_t)nframes * c->npup * c->npup * 4));
-  if (!use_fused(c) && c->active.size() > 0) {   // image-formation scratch of the general path
-    const long hll = (long)nframes * (long)c->active.size() * c->padded * c->padded;
-    HIPCHK(grow_scratch(c, c->d_hll, hll * sizeof(float)));
-  }
+// Bytes of each scratch buffer a call over nframes needs (0 when not needed): images when the
+// caller passes none but the general path or the correlation stage reads them, correlation images
+// likewise, the resampled phase grid, and the general path's hll.
+struct ScratchSizes {
+  size_t bimg = 0, corr = 0, interp = 0, hll = 0;
+};
+
+static ScratchSizes scratch_sizes(dasp_cent *c, int nframes, bool have_bimg, bool have_corr) {
+  ScratchSizes z;
+  const size_t F = (size_t)nframes;
+  if (!have_bimg && (!use_fused(c) || corr_stage(c))) z.bimg = F * c->nsubaps * c->nimg * c->nimg * 4;
+  if (!have_corr && c->correlationCentroiding && c->haveCorr) z.corr = F * c->nsubaps * c->corrsize * c->corrsize * 4;
+  if (c->interpolate) z.interp = F * c->npup * c->npup * 4;
+  if (!use_fused(c) && !c->active.empty()) z.hll = F * c->active.size() * c->padded * c->padded * 4;
+  return z;
+}
+
+// Scratch from the handle's own buffers (grown as needed).
+static int device_scratch(dasp_cent *c, int nframes, float *bimg, float *corr, Scratch &sc) {
+  const ScratchSizes z = scratch_sizes(c, nframes, bimg != nullptr, corr != nullptr);
+  if (z.bimg) HIPCHK(grow_scratch(c, c->d_bscratch, z.bimg));
+  if (z.corr) HIPCHK(grow_scratch(c, c->d_cscratch, z.corr));
+  if (z.interp) HIPCHK(grow_scratch(c, c->d_interp, z.interp));
+  if (z.hll) HIPCHK(grow_scratch(c, c->d_hll, z.hll));
+  sc.bimg = bimg ? bimg : (z.bimg ? c->d_bscratch.as<float>() : nullptr);
+  sc.corr = corr ? corr : (z.corr ? c->d_cscratch.as<float>() : nullptr);
+  sc.interp = z.interp ? c->d_interp.as<float>() : nullptr;
+  sc.hll = z.hll ? c->d_hll.as<float>() : nullptr;
+  return 0;
+}
+
+// Scratch carved from the fixed device array (256-byte aligned pieces); fails when it is too small.
+static int fixed_scratch(dasp_cent *c, int nframes, float *bimg, float *corr, Scratch &sc) {
+  const ScratchSizes z = scratch_sizes(c, nframes, bimg != nullptr, corr != nullptr);
+  auto up = [](size_t v) { return (v + 255) & ~(size_t)255; };
+  const size_t total = up(z.bimg) + up(z.corr) + up(z.interp) + up(z.hll);
+  if (total > kNoAllocScratchBytes)
+    return fail(DASP_CENT_EINVAL,
+                "run_device_noalloc: %d frames need %zu bytes of scratch, more than the fixed %zu "
+                "(pass image/correlation buffers or fewer frames)",
+                nframes, total, (size_t)kNoAllocScratchBytes);
+  if (total > 0 && !c->noallocScratch) return fail(DASP_CENT_EHIP, "run_device_noalloc: no scratch array on this device");
+  char *p = (char *)c->noallocScratch;
+  size_t off = 0;
+  auto take = [&](size_t nb) -> float * {
+    if (!nb) return nullptr;
+    float *q = (float *)(p + off);
+    off += up(nb);
+    return q;
+  };
+  sc.bimg = bimg ? bimg : take(z.bimg);
+  sc.corr = corr ? corr : take(z.corr);
+  sc.interp = take(z.interp);
+  sc.hll = take(z.hll);
   return 0;
 }
 
@@ -911,8 +968,8 @@ static int batch_layout(dasp_cent_batch *b) {
   }
   for (int i : b->others) {
     dasp_cent_batch_item &it = b->items[i];
-    float *bimg = it.bimg, *corr = it.corrimg;
-    int rc = device_scratch(it.handle, it.nframes, &bimg, &corr);
+    Scratch sc;
+    int rc = device_scratch(it.handle, it.nframes, it.bimg, it.corrimg, sc);
     if (rc) return rc;
   }
   // scratch addresses are final here: the graph captured next uses exactly these
@@ -931,12 +988,12 @@ static int batch_enqueue(dasp_cent_batch *b, hipStream_t st) {
   for (int i : b->others) {
     dasp_cent_batch_item &it = b->items[i];
     dasp_cent *c = it.handle;
-    float *bimg = it.bimg, *corr = it.corrimg;
-    int rc = device_scratch(c, it.nframes, &bimg, &corr);   // sized at layout time: no allocation here
+    Scratch sc;
+    int rc = device_scratch(c, it.nframes, it.bimg, it.corrimg, sc);   // sized at layout time: no allocation here
     if (rc) return rc;
     const bool timing = c->timing;
     c->timing = false;   // no event records inside a captured step
-    rc = launch_phase(c, it.phs, it.phs_stride, it.nframes, it.cents, bimg, corr, it.zpois, it.zread, st);
+    rc = launch_phase(c, it.phs, it.phs_stride, it.nframes, it.cents, sc.bimg, sc.corr, it.zpois, it.zread, sc, st);
     c->timing = timing;
     if (rc) return rc;
   }
@@ -1020,6 +1077,10 @@ int dasp_cent_initialise(const dasp_cent_params *pp, dasp_cent_t **out) {
     dasp_cent_free(c);
     return rc;
   };
+  {   // the fixed scratch array of dasp_cent_run_device_noalloc on this device (looked up once)
+    const hipError_t e = noalloc_scratch(&c->noallocScratch);
+    if (e != hipSuccess) return bail(fail(DASP_CENT_EHIP, "scratch array address: %s", hipGetErrorString(e)));
+  }
   c->p = p;
   c->nsubaps = p.nsubaps;
   c->nsubx = (int)std::sqrt((float)p.nsubaps);
@@ -1270,9 +1331,11 @@ int dasp_cent_run(dasp_cent_t *c, float *seconds) {
     const bool pois = need_pois(c), rd = need_read(c);
     int rc = draw_noise(c, pois, rd);
     if (rc) return rc;
-    rc = launch_phase(c, c->d_phs.as<float>(), 0, 1, c->d_cents.as<float>(), c->d_bimg.as<float>(),
-                      c->d_corrimg.as<float>(), pois ? c->d_zpois.as<float>() : nullptr,
-                      rd ? c->d_zread.as<float>() : nullptr, nullptr);
+    Scratch sc;
+    rc = device_scratch(c, 1, c->d_bimg.as<float>(), c->d_corrimg.as<float>(), sc);
+    if (rc) return rc;
+    rc = launch_phase(c, c->d_phs.as<float>(), 0, 1, c->d_cents.as<float>(), sc.bimg, sc.corr,
+                      pois ? c->d_zpois.as<float>() : nullptr, rd ? c->d_zread.as<float>() : nullptr, sc, nullptr);
     if (rc) return rc;
     HIPCHK(hipDeviceSynchronize());
     rc = copy_back(c);
@@ -1343,10 +1406,32 @@ int dasp_cent_run_device(dasp_cent_t *c, const float *d_phs, long phs_stride, in
                          void *stream) {
   int rc = check_device_run(c, d_phs, phs_stride, nframes, d_cents, d_zpois, d_zread);
   if (rc) return rc;
-  float *bimg = d_bimg, *corr = d_corrimg;
-  rc = device_scratch(c, nframes, &bimg, &corr);
+  Scratch sc;
+  rc = device_scratch(c, nframes, d_bimg, d_corrimg, sc);
   if (rc) return rc;
-  return launch_phase(c, d_phs, phs_stride, nframes, d_cents, bimg, corr, d_zpois, d_zread, (hipStream_t)stream);
+  return launch_phase(c, d_phs, phs_stride, nframes, d_cents, sc.bimg, sc.corr, d_zpois, d_zread, sc, (hipStream_t)stream);
+}
+
+int dasp_cent_run_device_noalloc(dasp_cent_t *c, const float *d_phs, long phs_stride, int nframes, float *d_cents,
+                                 float *d_bimg, float *d_corrimg, const float *d_zpois, const float *d_zread,
+                                 void *stream) {
+  int rc = check_device_run(c, d_phs, phs_stride, nframes, d_cents, d_zpois, d_zread);
+  if (rc) return rc;
+  Scratch sc;
+  rc = fixed_scratch(c, nframes, d_bimg, d_corrimg, sc);
+  if (rc) return rc;
+  // no device allocation below: a handle buffer that would have to grow fails instead, and no
+  // timing events are created
+  t_forbid_alloc = true;
+  t_alloc_refused = false;
+  const bool timing = c->timing;
+  c->timing = false;
+  rc = launch_phase(c, d_phs, phs_stride, nframes, d_cents, sc.bimg, sc.corr, d_zpois, d_zread, sc, (hipStream_t)stream);
+  c->timing = timing;
+  t_forbid_alloc = false;
+  if (rc && t_alloc_refused)
+    return fail(rc, "run_device_noalloc: a device buffer of the handle would have to grow (%s)", g_err.c_str());
+  return rc;
 }
 
 int dasp_cent_update_int(dasp_cent_t *c, int code, long v) {

@@ -7,6 +7,12 @@ namespace dasp {
 
 constexpr int kMaxSide = 64;  // largest psfsize/clipsize/nimg/corrsize handled on the device
 
+// Size of the fixed per-device scratch array of the allocation-free device run
+// (dasp_cent_run_device_noalloc, cent_noalloc.hip).
+constexpr size_t kNoAllocScratchBytes = (size_t)64 << 20;
+// Its address on the current device (looked up once per device).
+hipError_t noalloc_scratch(void **out);
+
 // Read of a read-only input at a wave-uniform index through the scalar cache (constant address
 // space). Such reads are counted by the scalar-memory counter, so waiting for them never waits
 // for the wave's outstanding vector loads (prefetches of the next item).

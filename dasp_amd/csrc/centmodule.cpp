@@ -15,7 +15,7 @@
 // Failures raise cent.error with the engine's message (the reference printed and returned NULL).
 //
 // Extensions (no counterpart in the reference): setForm, setFastTrig, setRelativeFloor, info,
-// runDevice, drawNormals, drawNormalsDevice, hcToSpectrum, setTiming, kernelTime, abiVersion and
+// runDevice, runDeviceNoAlloc, drawNormals, drawNormalsDevice, hcToSpectrum, setTiming, kernelTime, abiVersion and
 // the multi-WFS step batchCreate / batchTune / batchRun / batchInfo / batchSetConfig / batchFree.
 #define PY_SSIZE_T_CLEAN
 #include <Python.h>
@@ -350,6 +350,28 @@ static PyObject *py_runDevice(PyObject *, PyObject *args, PyObject *kw) {
   Py_RETURN_NONE;
 }
 
+static PyObject *py_runDeviceNoAlloc(PyObject *, PyObject *args, PyObject *kw) {
+  static const char *names[] = {"handle", "phs_ptr", "phs_stride", "nframes", "cents_ptr", "bimg_ptr", "corrimg_ptr",
+                                "zpois_ptr", "zread_ptr", "stream", nullptr};
+  PyObject *ho;
+  unsigned long long phs, cents, bimg = 0, corr = 0, zp = 0, zr = 0, st = 0;
+  long stride;
+  int nframes;
+  dasp_cent_t *h;
+  if (!PyArg_ParseTupleAndKeywords(args, kw, "OKliK|KKKKK", (char **)names, &ho, &phs, &stride, &nframes, &cents,
+                                   &bimg, &corr, &zp, &zr, &st) ||
+      !to_handle(ho, &h))
+    return nullptr;
+  int rc;
+  Py_BEGIN_ALLOW_THREADS
+    rc = dasp_cent_run_device_noalloc(h, (const float *)as_ptr(phs), stride, nframes, (float *)as_ptr(cents),
+                                      (float *)as_ptr(bimg), (float *)as_ptr(corr), (const float *)as_ptr(zp),
+                                      (const float *)as_ptr(zr), as_ptr(st));
+  Py_END_ALLOW_THREADS
+  if (rc != 0) return engine_error(rc);
+  Py_RETURN_NONE;
+}
+
 static PyObject *py_drawNormals(PyObject *, PyObject *args) {
   unsigned long long seed, stream;
   long n;
@@ -560,6 +582,8 @@ static PyMethodDef centMethods[] = {
      "runDevice(handle, phs_ptr, phs_stride, nframes, cents_ptr, bimg_ptr=0, corrimg_ptr=0, zpois_ptr=0, "
      "zread_ptr=0, stream=0): "
      "device-resident step over nframes frames (device addresses)."},
+    {"runDeviceNoAlloc", (PyCFunction)(void (*)(void))py_runDeviceNoAlloc, METH_VARARGS | METH_KEYWORDS,
+     "runDeviceNoAlloc(...): runDevice with its scratch in a fixed per-GPU array, no device allocation."},
     {"drawNormals", py_drawNormals, METH_VARARGS, "drawNormals(seed, stream, n) -> float32 N(0,1) array (PCG32 + Box-Muller)."},
     {"drawNormalsDevice", py_drawNormalsDevice, METH_VARARGS,
      "drawNormalsDevice(seed, stream, n, out_ptr, hip_stream=0): the same stream written on the device."},
@@ -600,5 +624,6 @@ PyMODINIT_FUNC PyInit_cent(void) {
   PyModule_AddIntConstant(m, "FORM_DFT_FP16", DASP_CENT_FORM_DFT_FP16);
   PyModule_AddIntConstant(m, "FORM_FFT32_GENERAL", DASP_CENT_FORM_FFT32_GENERAL);
   PyModule_AddIntConstant(m, "ABI_VERSION", DASP_CENT_ABI_VERSION);
+  PyModule_AddObject(m, "NOALLOC_SCRATCH_BYTES", PyLong_FromLong(DASP_CENT_NOALLOC_SCRATCH_BYTES));
   return m;
 }

@@ -184,6 +184,20 @@ int dasp_cent_run_device(dasp_cent_t *h, const float *d_phs, long phs_stride, in
                          float *d_cents, float *d_bimg, float *d_corrimg,
                          const float *d_zpois, const float *d_zread, void *stream);
 
+/* dasp_cent_run_device without any device allocation. No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc). Same arguments, kernels and results as dasp_cent_run_device;
+ * the scratch that call would allocate in the handle (images handed to the correlation stage or
+ * the general path when d_bimg is NULL, correlation images when d_corrimg is NULL, the resampled
+ * phase grid, the general path's image-formation buffer) comes from one fixed device array of
+ * DASP_CENT_NOALLOC_SCRATCH_BYTES per GPU, held until the process ends. A call needing more
+ * returns DASP_CENT_EINVAL (pass image buffers or fewer frames). Every call on a GPU shares that
+ * GPU's array: calls that overlap on one GPU (other streams, other threads, other handles) race. */
+#define DASP_CENT_NOALLOC_SCRATCH_BYTES (64L << 20)
+int dasp_cent_run_device_noalloc(dasp_cent_t *h, const float *d_phs, long phs_stride, int nframes,
+                                 float *d_cents, float *d_bimg, float *d_corrimg,
+                                 const float *d_zpois, const float *d_zread, void *stream);
+
 /* Host noise generator used by dasp_cent_run: n standard normals from (seed, stream), PCG32
  * (O'Neill 2014) uniforms through the Box-Muller transform. Deterministic and host only. */
 int dasp_cent_draw_normals(unsigned long seed, unsigned long stream, long n, float *out);
