@@ -16,7 +16,8 @@
 //
 // Extensions (no counterpart in the reference): setForm, setFastTrig, setRelativeFloor, info,
 // runDevice, runDeviceNoAlloc, drawNormals, drawNormalsDevice, hcToSpectrum, setTiming, kernelTime, abiVersion and
-// the multi-WFS step batchCreate / batchTune / batchRun / batchInfo / batchSetConfig / batchFree.
+// the multi-WFS step batchCreate / batchTune / batchRun / batchInfo / batchSetConfig / batchFree,
+// and the device reconstructor step reconCreate / reconSetGains / reconRun / reconFree.
 #define PY_SSIZE_T_CLEAN
 #include <Python.h>
 #define NPY_NO_DEPRECATED_API NPY_1_7_API_VERSION
@@ -566,6 +567,81 @@ static PyObject *py_batchFree(PyObject *, PyObject *args) {
   Py_RETURN_NONE;
 }
 
+// ---- reconstructor step ---------------------------------------------------------------------
+static PyObject *recon_error(int rc) {
+  const char *m = dasp_recon_last_error();
+  PyErr_SetString(CentError, (m && m[0]) ? m : "recon error");
+  (void)rc;
+  return nullptr;
+}
+
+static PyObject *py_reconCreate(PyObject *, PyObject *args) {
+  int ncents, nacts, layout;
+  unsigned long long mx;
+  long ld;
+  PyObject *io;
+  if (!PyArg_ParseTuple(args, "iiiKlO", &ncents, &nacts, &layout, &mx, &ld, &io)) return nullptr;
+  dasp_cent_array idx;
+  if (to_array(io, &idx, 'i', "index", false)) return nullptr;
+  if (idx.ndim != 1 || idx.dims[0] != ncents) {
+    PyErr_SetString(CentError, "recon: index must be int32 of length ncents");
+    return nullptr;
+  }
+  dasp_recon_t *r = nullptr;
+  const int rc = dasp_recon_create(ncents, nacts, layout, (const float *)as_ptr(mx), ld, (const int *)idx.data, &r);
+  if (rc != 0) return recon_error(rc);
+  return PyLong_FromVoidPtr(r);
+}
+
+// gains / decay: a float for every actuator, or a float32 array of nacts
+static PyObject *py_reconSetGains(PyObject *, PyObject *args) {
+  PyObject *ro, *go, *dobj;
+  if (!PyArg_ParseTuple(args, "OOO", &ro, &go, &dobj)) return nullptr;
+  dasp_recon_t *r = (dasp_recon_t *)PyLong_AsVoidPtr(ro);
+  if (PyErr_Occurred()) return nullptr;
+  dasp_cent_array ga, da;
+  float g = 1.f, d = 1.f;
+  std::memset(&ga, 0, sizeof(ga));
+  std::memset(&da, 0, sizeof(da));
+  if (PyArray_Check(go)) {
+    if (to_array(go, &ga, 'f', "gains", false)) return nullptr;
+  } else {
+    g = (float)PyFloat_AsDouble(go);
+    if (PyErr_Occurred()) return nullptr;
+  }
+  if (PyArray_Check(dobj)) {
+    if (to_array(dobj, &da, 'f', "decay", false)) return nullptr;
+  } else {
+    d = (float)PyFloat_AsDouble(dobj);
+    if (PyErr_Occurred()) return nullptr;
+  }
+  const int rc = dasp_recon_set_gains(r, (const float *)ga.data, g, (const float *)da.data, d);
+  if (rc != 0) return recon_error(rc);
+  Py_RETURN_NONE;
+}
+
+static PyObject *py_reconRun(PyObject *, PyObject *args) {
+  PyObject *ro;
+  unsigned long long sl, ac, st = 0;
+  long ss, as;
+  int nframes;
+  if (!PyArg_ParseTuple(args, "OKliKl|K", &ro, &sl, &ss, &nframes, &ac, &as, &st)) return nullptr;
+  dasp_recon_t *r = (dasp_recon_t *)PyLong_AsVoidPtr(ro);
+  if (PyErr_Occurred()) return nullptr;
+  const int rc = dasp_recon_run(r, (const float *)as_ptr(sl), ss, nframes, (float *)as_ptr(ac), as, as_ptr(st));
+  if (rc != 0) return recon_error(rc);
+  Py_RETURN_NONE;
+}
+
+static PyObject *py_reconFree(PyObject *, PyObject *args) {
+  PyObject *ro;
+  if (!PyArg_ParseTuple(args, "O", &ro)) return nullptr;
+  dasp_recon_t *r = (dasp_recon_t *)PyLong_AsVoidPtr(ro);
+  if (PyErr_Occurred()) return nullptr;
+  dasp_recon_free(r);
+  Py_RETURN_NONE;
+}
+
 static PyMethodDef centMethods[] = {
     {"run", py_run, METH_VARARGS, "Run the phase -> centroid algorithm; returns the elapsed seconds."},
     {"runSlope", py_runSlope, METH_VARARGS, "Run the image -> centroid algorithm."},
@@ -597,6 +673,12 @@ static PyMethodDef centMethods[] = {
     {"batchInfo", py_batchInfo, METH_VARARGS, "batchInfo(batch) -> (list of launch dicts, WFS outside them)."},
     {"batchSetConfig", py_batchSetConfig, METH_VARARGS, "batchSetConfig(batch, i, waves_per_wg, workgroups, chunk=0)."},
     {"batchFree", py_batchFree, METH_VARARGS, "batchFree(batch)."},
+    {"reconCreate", py_reconCreate, METH_VARARGS,
+     "reconCreate(ncents, nacts, layout, reconmx_ptr, ld, index) -> handle of the device reconstructor step."},
+    {"reconSetGains", py_reconSetGains, METH_VARARGS, "reconSetGains(r, gains, decay): floats or float32 arrays of nacts."},
+    {"reconRun", py_reconRun, METH_VARARGS,
+     "reconRun(r, slopes_ptr, slope_stride, nframes, acts_ptr, acts_stride, stream=0): acts = decay*acts - gains*(data.reconmx)."},
+    {"reconFree", py_reconFree, METH_VARARGS, "reconFree(r)."},
     {nullptr, nullptr, 0, nullptr}};
 
 static struct PyModuleDef centModule = {PyModuleDef_HEAD_INIT, "cent",

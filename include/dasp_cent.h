@@ -274,6 +274,31 @@ int dasp_cent_batch_others(dasp_cent_batch_t *b);
 
 int dasp_cent_batch_free(dasp_cent_batch_t *b);
 
+/* ---- reconstructor step on the device (science/tomoRecon.py, dense reconmx) ------------------
+ * The slopes never leave HBM: the slope vector of the reconstructor (length ncents: the x slopes of
+ * every active sub-aperture of every WFS, then the y slopes; centIndex, science/tomoRecon.py:131-150,
+ * getInput :682-700) is gathered on the device from a slope buffer (index[c] = position of entry c
+ * within one frame of it, e.g. WFS k's slopes at a per-WFS offset), and each frame's actuator vector
+ * is updated as
+ *     acts = decay * acts - gains * (data . reconmx)        (:1140-1142, :1174)
+ * layout 0: reconmx (ncents, nmodes) with row stride ld >= nacts, as science/tomoRecon.py's
+ * dot(data, reconmx); layout 1: (nmodes, ncents) with ld >= ncents, its dot(reconmx, data). Only the
+ * first nacts modes are formed. The matrix is device memory borrowed from the caller. */
+typedef struct dasp_recon dasp_recon_t;
+const char *dasp_recon_last_error(void);
+int dasp_recon_create(int ncents, int nacts, int layout, const float *d_reconmx, long ld, const int *h_index,
+                      dasp_recon_t **out);
+/* Per-actuator gains and decay (host arrays of nacts, copied) or, for a NULL array, one value for
+ * every actuator. Defaults: gain 1, decay 1. */
+int dasp_recon_set_gains(dasp_recon_t *r, const float *h_gains, float gain, const float *h_decay, float decay);
+/* nframes independent slope vectors: frame f reads d_slopes + f*slope_stride (slope_stride, the
+ * floats per frame of the slope buffer, must exceed every index) and updates d_acts + f*acts_stride
+ * (acts_stride >= nacts when nframes > 1). The matrix must hold (layout 0) ncents*ld or (layout 1)
+ * nacts*ld floats. Asynchronous on stream; deterministic (fixed summation order). */
+int dasp_recon_run(dasp_recon_t *r, const float *d_slopes, long slope_stride, int nframes, float *d_acts,
+                   long acts_stride, void *stream);
+int dasp_recon_free(dasp_recon_t *r);
+
 #ifdef __cplusplus
 }
 #endif
