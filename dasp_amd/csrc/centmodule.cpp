@@ -286,11 +286,12 @@ static PyObject *py_testcorrelation(PyObject *, PyObject *args) {
 }
 
 // ---- extensions -------------------------------------------------------------------------
-static PyObject *handle_int(PyObject *args, int (*fn)(dasp_cent_t *, int), int dflt) {
+static PyObject *handle_int(PyObject *args, PyObject *kw, int (*fn)(dasp_cent_t *, int), int dflt) {
+  static const char *names[] = {"handle", "enable", nullptr};
   PyObject *ho;
   int v = dflt;
   dasp_cent_t *h;
-  if (!PyArg_ParseTuple(args, "O|p", &ho, &v) || !to_handle(ho, &h)) return nullptr;
+  if (!PyArg_ParseTupleAndKeywords(args, kw, "O|p", (char **)names, &ho, &v) || !to_handle(ho, &h)) return nullptr;
   CHECK(fn(h, v));
   Py_RETURN_NONE;
 }
@@ -303,9 +304,13 @@ static PyObject *py_setForm(PyObject *, PyObject *args) {
   CHECK(dasp_cent_set_form(h, form));
   Py_RETURN_NONE;
 }
-static PyObject *py_setFastTrig(PyObject *, PyObject *args) { return handle_int(args, dasp_cent_set_fast_trig, 1); }
-static PyObject *py_setRelativeFloor(PyObject *, PyObject *args) { return handle_int(args, dasp_cent_set_relative_floor, 1); }
-static PyObject *py_setTiming(PyObject *, PyObject *args) { return handle_int(args, dasp_cent_set_timing, 1); }
+static PyObject *py_setFastTrig(PyObject *, PyObject *args, PyObject *kw) {
+  return handle_int(args, kw, dasp_cent_set_fast_trig, 1);
+}
+static PyObject *py_setRelativeFloor(PyObject *, PyObject *args, PyObject *kw) {
+  return handle_int(args, kw, dasp_cent_set_relative_floor, 1);
+}
+static PyObject *py_setTiming(PyObject *, PyObject *args, PyObject *kw) { return handle_int(args, kw, dasp_cent_set_timing, 1); }
 
 static PyObject *py_info(PyObject *, PyObject *args) {
   PyObject *ho;
@@ -316,11 +321,12 @@ static PyObject *py_info(PyObject *, PyObject *args) {
   return Py_BuildValue("(iii)", a, b, c);
 }
 
-static PyObject *py_kernelTime(PyObject *, PyObject *args) {
+static PyObject *py_kernelTime(PyObject *, PyObject *args, PyObject *kw) {
+  static const char *names[] = {"handle", "reset", nullptr};
   PyObject *ho;
   int reset = 0;
   dasp_cent_t *h;
-  if (!PyArg_ParseTuple(args, "O|p", &ho, &reset) || !to_handle(ho, &h)) return nullptr;
+  if (!PyArg_ParseTupleAndKeywords(args, kw, "O|p", (char **)names, &ho, &reset) || !to_handle(ho, &h)) return nullptr;
   double ms = 0.0;
   long n = 0;
   CHECK(dasp_cent_kernel_time(h, reset, &ms, &n));
@@ -393,10 +399,11 @@ static PyObject *py_drawNormals(PyObject *, PyObject *args) {
   return out;
 }
 
-static PyObject *py_drawNormalsDevice(PyObject *, PyObject *args) {
+static PyObject *py_drawNormalsDevice(PyObject *, PyObject *args, PyObject *kw) {
+  static const char *names[] = {"seed", "stream", "n", "out_ptr", "hip_stream", nullptr};
   unsigned long long seed, stream, out, st = 0;
   long n;
-  if (!PyArg_ParseTuple(args, "KKlK|K", &seed, &stream, &n, &out, &st)) return nullptr;
+  if (!PyArg_ParseTupleAndKeywords(args, kw, "KKlK|K", (char **)names, &seed, &stream, &n, &out, &st)) return nullptr;
   CHECK(dasp_cent_draw_normals_device((unsigned long)seed, (unsigned long)stream, n, (float *)as_ptr(out), as_ptr(st)));
   Py_RETURN_NONE;
 }
@@ -490,12 +497,13 @@ static bool to_batch(PyObject *o, dasp_cent_batch_t **b) {
   return !PyErr_Occurred();
 }
 
-static PyObject *py_batchTune(PyObject *, PyObject *args) {
+static PyObject *py_batchTune(PyObject *, PyObject *args, PyObject *kw) {
+  static const char *names[] = {"batch", "stream", "reps", nullptr};
   PyObject *bo;
   unsigned long long st = 0;
   int reps = 5;
   dasp_cent_batch_t *b;
-  if (!PyArg_ParseTuple(args, "O|Ki", &bo, &st, &reps) || !to_batch(bo, &b)) return nullptr;
+  if (!PyArg_ParseTupleAndKeywords(args, kw, "O|Ki", (char **)names, &bo, &st, &reps) || !to_batch(bo, &b)) return nullptr;
   int rc;
   Py_BEGIN_ALLOW_THREADS
     rc = dasp_cent_batch_tune(b, as_ptr(st), reps);
@@ -504,11 +512,12 @@ static PyObject *py_batchTune(PyObject *, PyObject *args) {
   Py_RETURN_NONE;
 }
 
-static PyObject *py_batchRun(PyObject *, PyObject *args) {
+static PyObject *py_batchRun(PyObject *, PyObject *args, PyObject *kw) {
+  static const char *names[] = {"batch", "stream", nullptr};
   PyObject *bo;
   unsigned long long st = 0;
   dasp_cent_batch_t *b;
-  if (!PyArg_ParseTuple(args, "O|K", &bo, &st) || !to_batch(bo, &b)) return nullptr;
+  if (!PyArg_ParseTupleAndKeywords(args, kw, "O|K", (char **)names, &bo, &st) || !to_batch(bo, &b)) return nullptr;
   CHECK(dasp_cent_batch_run(b, as_ptr(st)));
   Py_RETURN_NONE;
 }
@@ -549,12 +558,14 @@ static PyObject *py_batchInfo(PyObject *, PyObject *args) {
   return Py_BuildValue("(Ni)", lst, others);
 }
 
-static PyObject *py_batchSetConfig(PyObject *, PyObject *args) {
+static PyObject *py_batchSetConfig(PyObject *, PyObject *args, PyObject *kw) {
+  static const char *names[] = {"batch", "i", "waves_per_wg", "workgroups", "chunk", nullptr};
   PyObject *bo;
   int i, waves, chunk = 0;
   long wgs;
   dasp_cent_batch_t *b;
-  if (!PyArg_ParseTuple(args, "Oiil|i", &bo, &i, &waves, &wgs, &chunk) || !to_batch(bo, &b)) return nullptr;
+  if (!PyArg_ParseTupleAndKeywords(args, kw, "Oiil|i", (char **)names, &bo, &i, &waves, &wgs, &chunk) || !to_batch(bo, &b))
+    return nullptr;
   CHECK(dasp_cent_batch_set_config(b, i, waves, wgs, chunk));
   Py_RETURN_NONE;
 }
@@ -620,12 +631,13 @@ static PyObject *py_reconSetGains(PyObject *, PyObject *args) {
   Py_RETURN_NONE;
 }
 
-static PyObject *py_reconRun(PyObject *, PyObject *args) {
+static PyObject *py_reconRun(PyObject *, PyObject *args, PyObject *kw) {
+  static const char *names[] = {"recon", "slopes_ptr", "slope_stride", "nframes", "acts_ptr", "acts_stride", "stream", nullptr};
   PyObject *ro;
   unsigned long long sl, ac, st = 0;
   long ss, as;
   int nframes;
-  if (!PyArg_ParseTuple(args, "OKliKl|K", &ro, &sl, &ss, &nframes, &ac, &as, &st)) return nullptr;
+  if (!PyArg_ParseTupleAndKeywords(args, kw, "OKliKl|K", (char **)names, &ro, &sl, &ss, &nframes, &ac, &as, &st)) return nullptr;
   dasp_recon_t *r = (dasp_recon_t *)PyLong_AsVoidPtr(ro);
   if (PyErr_Occurred()) return nullptr;
   const int rc = dasp_recon_run(r, (const float *)as_ptr(sl), ss, nframes, (float *)as_ptr(ac), as, as_ptr(st));
@@ -650,8 +662,8 @@ static PyMethodDef centMethods[] = {
     {"free", py_free, METH_VARARGS, "Free a handle (invalidates it)."},
     {"testcorrelation", py_testcorrelation, METH_VARARGS, "testcorrelation(img, corr): corr <- correlation(img, corr)."},
     {"setForm", py_setForm, METH_VARARGS, "setForm(handle, form): FORM_FFT32, FORM_DFT_BF16, FORM_DFT_FP16 or FORM_FFT32_GENERAL."},
-    {"setFastTrig", py_setFastTrig, METH_VARARGS, "setFastTrig(handle, enable=True): reduced-argument hardware sin/cos."},
-    {"setRelativeFloor", py_setRelativeFloor, METH_VARARGS,
+    {"setFastTrig", (PyCFunction)(void (*)(void))py_setFastTrig, METH_VARARGS | METH_KEYWORDS, "setFastTrig(handle, enable=True): reduced-argument hardware sin/cos."},
+    {"setRelativeFloor", (PyCFunction)(void (*)(void))py_setRelativeFloor, METH_VARARGS | METH_KEYWORDS,
      "setRelativeFloor(handle, enable=True): threshType 0 floor = noiseFloor * max(image) when 0 < noiseFloor <= 1."},
     {"info", py_info, METH_VARARGS, "info(handle) -> (nactive, npup, corrsize)."},
     {"runDevice", (PyCFunction)(void (*)(void))py_runDevice, METH_VARARGS | METH_KEYWORDS,
@@ -661,22 +673,22 @@ static PyMethodDef centMethods[] = {
     {"runDeviceNoAlloc", (PyCFunction)(void (*)(void))py_runDeviceNoAlloc, METH_VARARGS | METH_KEYWORDS,
      "runDeviceNoAlloc(...): runDevice with its scratch in a fixed per-GPU array, no device allocation."},
     {"drawNormals", py_drawNormals, METH_VARARGS, "drawNormals(seed, stream, n) -> float32 N(0,1) array (PCG32 + Box-Muller)."},
-    {"drawNormalsDevice", py_drawNormalsDevice, METH_VARARGS,
+    {"drawNormalsDevice", (PyCFunction)(void (*)(void))py_drawNormalsDevice, METH_VARARGS | METH_KEYWORDS,
      "drawNormalsDevice(seed, stream, n, out_ptr, hip_stream=0): the same stream written on the device."},
     {"hcToSpectrum", py_hcToSpectrum, METH_VARARGS, "hcToSpectrum(pattern_hc) -> complex spectrum of a half-complex pattern."},
-    {"setTiming", py_setTiming, METH_VARARGS, "setTiming(handle, enable=True): HIP events around the image kernel."},
-    {"kernelTime", py_kernelTime, METH_VARARGS, "kernelTime(handle, reset=False) -> (ms, launches)."},
+    {"setTiming", (PyCFunction)(void (*)(void))py_setTiming, METH_VARARGS | METH_KEYWORDS, "setTiming(handle, enable=True): HIP events around the image kernel."},
+    {"kernelTime", (PyCFunction)(void (*)(void))py_kernelTime, METH_VARARGS | METH_KEYWORDS, "kernelTime(handle, reset=False) -> (ms, launches)."},
     {"abiVersion", py_abiVersion, METH_NOARGS, "DASP_CENT_ABI_VERSION of the engine objects."},
     {"batchCreate", py_batchCreate, METH_VARARGS, "batchCreate(items) -> batch: multi-WFS step over dicts of device buffers."},
-    {"batchTune", py_batchTune, METH_VARARGS, "batchTune(batch, stream=0, reps=5): autotune the persistent launches."},
-    {"batchRun", py_batchRun, METH_VARARGS, "batchRun(batch, stream=0): one step (one graph launch)."},
+    {"batchTune", (PyCFunction)(void (*)(void))py_batchTune, METH_VARARGS | METH_KEYWORDS, "batchTune(batch, stream=0, reps=5): autotune the persistent launches."},
+    {"batchRun", (PyCFunction)(void (*)(void))py_batchRun, METH_VARARGS | METH_KEYWORDS, "batchRun(batch, stream=0): one step (one graph launch)."},
     {"batchInfo", py_batchInfo, METH_VARARGS, "batchInfo(batch) -> (list of launch dicts, WFS outside them)."},
-    {"batchSetConfig", py_batchSetConfig, METH_VARARGS, "batchSetConfig(batch, i, waves_per_wg, workgroups, chunk=0)."},
+    {"batchSetConfig", (PyCFunction)(void (*)(void))py_batchSetConfig, METH_VARARGS | METH_KEYWORDS, "batchSetConfig(batch, i, waves_per_wg, workgroups, chunk=0)."},
     {"batchFree", py_batchFree, METH_VARARGS, "batchFree(batch)."},
     {"reconCreate", py_reconCreate, METH_VARARGS,
      "reconCreate(ncents, nacts, layout, reconmx_ptr, ld, index) -> handle of the device reconstructor step."},
     {"reconSetGains", py_reconSetGains, METH_VARARGS, "reconSetGains(r, gains, decay): floats or float32 arrays of nacts."},
-    {"reconRun", py_reconRun, METH_VARARGS,
+    {"reconRun", (PyCFunction)(void (*)(void))py_reconRun, METH_VARARGS | METH_KEYWORDS,
      "reconRun(r, slopes_ptr, slope_stride, nframes, acts_ptr, acts_stride, stream=0): acts = decay*acts - gains*(data.reconmx)."},
     {"reconFree", py_reconFree, METH_VARARGS, "reconFree(r)."},
     {nullptr, nullptr, 0, nullptr}};

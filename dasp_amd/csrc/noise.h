@@ -30,20 +30,29 @@ __device__ inline float poisson_from_normal(float lam, float z) {
     const float u = 0.5f * erfcf(-z * 0.70710678118654752f);
     float p = expf(-lam);
     float F = p;
-    // Every lane still searching has taken exactly i steps at step i, so the step count is
-    // wave-uniform and 1/(k+1) comes from the table through the scalar cache; the loop runs
-    // until the wave's last lane stops, so its body is kept short.
-    int k = 0;
-    for (int i = 0; i < 40; ++i) {
-      if (!(u > F)) break;
-      const float pn = p * (lam * kPoisInv[i]);
-      const float Fn = F + pn;
-      if (!(Fn > F)) break;
-      k = i + 1;
-      p = pn;
-      F = Fn;
+    // k = the number of leading steps i (from 1) at which u > F(i-1) and the sum still grows,
+    // F(i) > F(i-1). Both conditions are monotone: once u <= F it stays so (F never decreases),
+    // and the sum can stop growing only past the mode, where the terms keep shrinking. So k is
+    // counted without branches: a lane's flag drops at its first failing step and freezes its
+    // count, the step count is wave-uniform (1/(k+1) comes from the table through the scalar
+    // cache) and the wave leaves the loop, checked every four steps, once no lane is searching.
+    // The arithmetic of each step is that of the plain search: the same k, bit for bit.
+    float k = 0.f;
+    bool go = u > F;
+    for (int i = 0; i < 40; i += 4) {
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        const float pn = p * (lam * kPoisInv[i + t]);
+        const float Fn = F + pn;
+        go = go && (Fn > F);
+        k += go ? 1.f : 0.f;
+        go = go && (u > Fn);
+        p = pn;
+        F = Fn;
+      }
+      if (!__any(go)) break;
     }
-    return (float)k;
+    return k;
   }
   const float s = sqrtf(lam + 1.0f / 12.0f);
   const float s2 = s * s;
