@@ -2,8 +2,9 @@
 //
 // Forward transform X[k] = sum_i x[i] exp(-2 pi i k i / N), unnormalised (the FFTW_FORWARD
 // convention of the reference, cmod/src/centmodule.c:3105). Radix-2 decimation in time; every
-// twiddle is a compile-time constant and the trivial ones (1, -i, (1-i)/sqrt2, ...) are applied
-// without multiplications. The index permutations are register renames and cost nothing.
+// twiddle is a compile-time constant, the trivial ones (1, -1, -i, +i) are applied by renames and
+// the others inside the butterfly as fused multiply-adds (butterfly below). The index
+// permutations are register renames and cost nothing.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <type_traits>
@@ -89,6 +90,43 @@ __device__ __forceinline__ cf twiddle(cf a) {
   }
 }
 
+constexpr double cabsd(double v) { return v < 0 ? -v : v; }
+
+// Radix-2 butterfly u <- u + w t, t <- u - w t with w = exp(-2 pi i J / M). Trivial twiddles
+// (1, -1, -i, +i) are exact renames; the others use the scaled rotation
+//   w t = c ((t.x - g t.y) + i (t.y + g t.x)),  g = s / c     (|c| >= |s|)
+//   w t = s ((h t.x - t.y) + i (h t.y + t.x)),  h = c / s     (|s| >  |c|)
+// (w = c + i s), so that each output component is one fused multiply-add: six instructions per
+// butterfly instead of eight (complex product, then add and subtract).
+template <int J, int M>
+__device__ __forceinline__ void butterfly(cf &u, cf &t) {
+  constexpr int j = ((J % M) + M) % M;
+  if constexpr (j == 0 || 2 * j == M || 4 * j == M || 4 * j == 3 * M) {
+    const cf wt = twiddle<j, M>(t);
+    const cf a = u;
+    u = cadd(a, wt);
+    t = csub(a, wt);
+  } else {
+    constexpr double th = -2.0 * ct::PI * j / M;
+    constexpr double cd = ct::cos(th), sd = ct::sin(th);
+    float vx, vy, m;
+    if constexpr (cabsd(cd) >= cabsd(sd)) {
+      constexpr float g = (float)(sd / cd);
+      m = (float)cd;
+      vx = fmaf(-g, t.y, t.x);
+      vy = fmaf(g, t.x, t.y);
+    } else {
+      constexpr float h = (float)(cd / sd);
+      m = (float)sd;
+      vx = fmaf(h, t.x, -t.y);
+      vy = fmaf(h, t.y, t.x);
+    }
+    const cf a = u;
+    u = {fmaf(m, vx, a.x), fmaf(m, vy, a.y)};
+    t = {fmaf(-m, vx, a.x), fmaf(-m, vy, a.y)};
+  }
+}
+
 template <int N>
 constexpr int ilog2() {
   int l = 0;
@@ -119,10 +157,7 @@ __device__ __forceinline__ void fft_fwd(cf (&x)[N]) {
         constexpr int k = decltype(g)::value * m;
         static_for<0, h>([&](auto jj) {
           constexpr int j = decltype(jj)::value;
-          cf t = twiddle<j, m>(y[k + j + h]);
-          cf u = y[k + j];
-          y[k + j] = cadd(u, t);
-          y[k + j + h] = csub(u, t);
+          butterfly<j, m>(y[k + j], y[k + j + h]);
         });
       });
     });

@@ -293,7 +293,8 @@ __device__ __forceinline__ void fused_item(const FusedArgs &a, const ItemRef &it
           if constexpr (r == 0) zs[l].x = cabs2(y2[decltype(qq)::value]);
           else zs[l].y = cabs2(y2[decltype(qq)::value]);
         } else {
-          acc[ib][jb] += cabs2(y2[decltype(qq)::value]);
+          const cf v = y2[decltype(qq)::value];
+          acc[ib][jb] = fmaf(v.y, v.y, fmaf(v.x, v.x, acc[ib][jb]));   // |X|^2 binned: two fmas
         }
       });
     });
