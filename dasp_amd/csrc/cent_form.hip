@@ -67,7 +67,7 @@ __global__ __launch_bounds__((64 * FormShape<NP, F>::W)) void form_hll_kernel(Fo
   cf x[NP];
   load_column<NP, F>(ph, am, pu, a.tilt, n, a.npup, j, valid, x);
   cf z[F];
-  fft_fwd_padded<NP, F>(x, z);
+  fft_fwd_zp<NP, F>(x, z);
   cf *tb = &tbuf[w][sl * F * TS];
   static_for<0, F>([&](auto kk) {
     constexpr int k = decltype(kk)::value;
@@ -82,7 +82,7 @@ __global__ __launch_bounds__((64 * FormShape<NP, F>::W)) void form_hll_kernel(Fo
     cf row[NP];
     static_for<0, NP>([&](auto jj) { row[decltype(jj)::value] = tb[k * TS + decltype(jj)::value]; });
     cf X[F];
-    fft_fwd_padded<NP, F>(row, X);
+    fft_fwd_zp<NP, F>(row, X);
     if (!valid) return;
     if (Q <= F) {
       const int o = (F - Q) >> 1;

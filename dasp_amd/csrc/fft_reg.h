@@ -140,7 +140,25 @@ constexpr int bitrev(int v) {
   return r;
 }
 
-// In-place forward FFT of length N (power of two, N <= 64) on x[0..N-1].
+// Decimation-in-time stages S0..log2(N) on y (bit-reversed input order); stage s combines
+// pairs of 2^(s-1)-point transforms into 2^s-point ones.
+template <int N, int S0>
+__device__ __forceinline__ void dit_stages(cf (&y)[N]) {
+  constexpr int L = ilog2<N>();
+  static_for<S0, L + 1>([&](auto s) {
+    constexpr int m = 1 << decltype(s)::value;
+    constexpr int h = m / 2;
+    static_for<0, N / m>([&](auto g) {
+      constexpr int k = decltype(g)::value * m;
+      static_for<0, h>([&](auto jj) {
+        constexpr int j = decltype(jj)::value;
+        butterfly<j, m>(y[k + j], y[k + j + h]);
+      });
+    });
+  });
+}
+
+// In-place forward FFT of length N (power of two, N <= 128) on x[0..N-1].
 template <int N>
 __device__ __forceinline__ void fft_fwd(cf (&x)[N]) {
   static_assert((N & (N - 1)) == 0 && N >= 1 && N <= 128, "power-of-two length");
@@ -150,19 +168,28 @@ __device__ __forceinline__ void fft_fwd(cf (&x)[N]) {
     constexpr int L = ilog2<N>();
     cf y[N];
     static_for<0, N>([&](auto i) { y[bitrev<L>(decltype(i)::value)] = x[decltype(i)::value]; });
-    static_for<1, L + 1>([&](auto s) {
-      constexpr int m = 1 << decltype(s)::value;
-      constexpr int h = m / 2;
-      static_for<0, N / m>([&](auto g) {
-        constexpr int k = decltype(g)::value * m;
-        static_for<0, h>([&](auto jj) {
-          constexpr int j = decltype(jj)::value;
-          butterfly<j, m>(y[k + j], y[k + j + h]);
-        });
-      });
-    });
+    dit_stages<N, 1>(y);
     static_for<0, N>([&](auto i) { x[decltype(i)::value] = y[decltype(i)::value]; });
   }
+}
+
+// Zero-padded forward FFT by decimation in time: x holds the NP leading inputs of an F-point
+// transform whose other inputs are zero (F = R*NP, R a power of two);
+//   y[k] = sum_{i<NP} x[i] exp(-2 pi i k i / F),  k < F.
+// In bit-reversed order the first log2(R) stages see one nonzero input per group of R, so after
+// them each group holds that input R times and nothing is computed; the remaining stages run as
+// in fft_fwd. Cheaper than R transforms of NP points on pre-twiddled inputs: the
+// twiddles sit inside the scaled butterflies (324 instead of 352 instructions for 16 -> 32).
+template <int NP, int F>
+__device__ __forceinline__ void fft_fwd_zp(const cf (&x)[NP], cf (&y)[F]) {
+  constexpr int R = F / NP, L = ilog2<F>(), LR = ilog2<R>();
+  static_assert(R * NP == F && (R & (R - 1)) == 0 && (F & (F - 1)) == 0, "power-of-two lengths");
+  static_for<0, NP>([&](auto gg) {
+    constexpr int g = decltype(gg)::value;
+    constexpr int i0 = bitrev<L>(g * R);   // < NP: the low log2(R) bits of g*R are zero
+    static_for<0, R>([&](auto t) { y[g * R + decltype(t)::value] = x[i0]; });
+  });
+  dit_stages<F, LR + 1>(y);
 }
 
 // In-place unnormalised inverse FFT, X[k] = sum_i x[i] exp(+2 pi i k i / N) (the FFTW_BACKWARD
@@ -173,25 +200,6 @@ __device__ __forceinline__ void fft_inv(cf (&x)[N]) {
   static_for<0, N>([&](auto i) { x[decltype(i)::value] = {x[decltype(i)::value].y, x[decltype(i)::value].x}; });
   fft_fwd<N>(x);
   static_for<0, N>([&](auto i) { x[decltype(i)::value] = {x[decltype(i)::value].y, x[decltype(i)::value].x}; });
-}
-
-// Zero-padded transform: x holds NP inputs (x[i], i<NP; the rest of the F-point input is zero);
-// out[k] = sum_{i<NP} x[i] exp(-2 pi i k i / F) for k < F, with F = R*NP.
-// out[R*q + r] = FFT_NP(x[i] * w_F^{i r})[q].
-template <int NP, int F>
-__device__ __forceinline__ void fft_fwd_padded(const cf (&x)[NP], cf (&out)[F]) {
-  constexpr int R = F / NP;
-  static_assert(R * NP == F && R >= 1, "F must be a multiple of NP");
-  static_for<0, R>([&](auto rr) {
-    constexpr int r = decltype(rr)::value;
-    cf y[NP];
-    static_for<0, NP>([&](auto ii) {
-      constexpr int i = decltype(ii)::value;
-      y[i] = twiddle<i * r, F>(x[i]);
-    });
-    fft_fwd<NP>(y);
-    static_for<0, NP>([&](auto q) { out[R * decltype(q)::value + r] = y[decltype(q)::value]; });
-  });
 }
 
 }  // namespace dasp

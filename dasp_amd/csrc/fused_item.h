@@ -253,50 +253,36 @@ __device__ __forceinline__ void fused_item(const FusedArgs &a, const ItemRef &it
     x[i] = {col.pu[i] * cs, col.pu[i] * sn};
   });
 
-  // ---- transpose in R phases + pass 2 + binning ----
-  // Phase r stages rows k = q*R + r (q < NP): pass-1 output Z[q*R + r][j] = FFT_NP(x[i] w_F^{ir})[q],
-  // so each phase needs one NP-point transform per lane and nothing else stays live.
+  // ---- pass 1, transpose in R phases, pass 2, binning ----
+  // Pass 1 is one zero-padded F-point transform of column j (Z[k][j], k < F). Phase r stages
+  // rows k = q*R + r (q < NP), so that only NP x NP values per subaperture sit in LDS at a time;
+  // lane j then transforms row k = j*R + r along its NP nonzero columns, zero-padded to F.
   float acc[RB][NIMG];
   static_for<0, RB>([&](auto ib) {
     static_for<0, NIMG>([&](auto jb) { acc[decltype(ib)::value][decltype(jb)::value] = 0.f; });
   });
   cf zs[SPOT ? F : 1];   // spot kernel: |X|^2 of rows 2j (re) and 2j+1 (im), unrotated
+  cf z1[F];
+  fft_fwd_zp<NP, F>(x, z1);
   static_for<0, R>([&](auto rr) {
     constexpr int r = decltype(rr)::value;
-    {
-      cf y[NP];
-      static_for<0, NP>([&](auto ii) {
-        constexpr int i = decltype(ii)::value;
-        y[i] = twiddle<i * r, F>(x[i]);
-      });
-      fft_fwd<NP>(y);
-      static_for<0, NP>([&](auto qq) { st[decltype(qq)::value * TS + j] = y[decltype(qq)::value]; });
-    }
+    static_for<0, NP>([&](auto qq) { st[decltype(qq)::value * TS + j] = z1[decltype(qq)::value * R + r]; });
     wave_lds_sync();
     cf row[NP];
     static_for<0, NP>([&](auto jj) { row[decltype(jj)::value] = st[j * TS + decltype(jj)::value]; });
     wave_lds_sync();
     constexpr int ib = R >= B ? r / B : 0;
-    // row k = j*R + r along its columns: X[R*q2 + r2] = FFT_NP(row[jj] w_F^{jj r2})[q2]
-    static_for<0, R>([&](auto rr2) {
-      constexpr int r2 = decltype(rr2)::value;
-      cf y2[NP];
-      static_for<0, NP>([&](auto jj) {
-        constexpr int jc = decltype(jj)::value;
-        y2[jc] = twiddle<jc * r2, F>(row[jc]);
-      });
-      fft_fwd<NP>(y2);
-      static_for<0, NP>([&](auto qq) {
-        constexpr int l = R * decltype(qq)::value + r2;
-        constexpr int jb = (F - 1 - l) / B;
-        if constexpr (SPOT) {
-          if constexpr (r == 0) zs[l].x = cabs2(y2[decltype(qq)::value]);
-          else zs[l].y = cabs2(y2[decltype(qq)::value]);
-        } else {
-          const cf v = y2[decltype(qq)::value];
-          acc[ib][jb] = fmaf(v.y, v.y, fmaf(v.x, v.x, acc[ib][jb]));   // |X|^2 binned: two fmas
-        }
-      });
+    cf y2[F];
+    fft_fwd_zp<NP, F>(row, y2);
+    static_for<0, F>([&](auto ll) {
+      constexpr int l = decltype(ll)::value;
+      constexpr int jb = (F - 1 - l) / B;
+      if constexpr (SPOT) {
+        if constexpr (r == 0) zs[l].x = cabs2(y2[l]);
+        else zs[l].y = cabs2(y2[l]);
+      } else {
+        acc[ib][jb] = fmaf(y2[l].y, y2[l].y, fmaf(y2[l].x, y2[l].x, acc[ib][jb]));   // |X|^2 binned: two fmas
+      }
     });
   });
   if constexpr (SPOT) spot_convolve<NP, F, B, RB, NIMG>(a, s, j, zs, st, acc);
@@ -319,26 +305,63 @@ __device__ __forceinline__ void fused_item(const FusedArgs &a, const ItemRef &it
   const int st0 = (NIMG - a.ncen) / 2, en0 = NIMG - st0;
   float sw = 0.f, sxw = 0.f, syw = 0.f;
   if (!a.slow_detector) {
-    // no noise, sky array, relative floor, power or weights: all inline
+    // no noise, sky array, relative floor, power or weights: all inline, the threshold mode and
+    // the image store decided once per item rather than per pixel
     const float add = (skyadd > 0.f ? skyadd : 0.f) + (a.threshType >= 0 && a.threshType <= 2 ? a.readbg : 0.f);
     const float thr = a.floor;
-    float *bout = a.bimg && owner ? a.bimg + (long)frame * a.bimg_stride + pix0 : nullptr;
-    static_for<0, RB>([&](auto ibb) {
-      constexpr int ib = decltype(ibb)::value;
-      const int I = det_row(ib);
-      static_for<0, NIMG>([&](auto jbb) {
-        constexpr int jb = decltype(jbb)::value;
-        float v = fmaf(acc[ib][jb], scale, 0.f) + add;
-        if (a.threshType == 1) v = v < thr ? 0.f : v;
-        else if (a.threshType == 0) v = v < thr ? 0.f : v - thr;
-        if (bout) bout[I * NIMG + jb] = v;
-        if (owner && I >= st0 && I < en0 && jb >= st0 && jb < en0) {
-          sw += v;
-          sxw += v * jb;
-          syw += v * I;
+    const int tt = a.threshType;
+    const bool full = st0 == 0;   // the CoG window is the whole image
+    // Every pixel is a sum of |X|^2 times a positive scale, so a zero floor (type 0) or a floor
+    // <= 0 (type 1) changes none. With nothing added either and no image returned, the CoG is
+    // that of the bins: the scale cancels in the ratio, and the sums are taken on the bins
+    // (for one detector row per lane; with more, the extra path costs registers).
+    const bool thr_noop = tt < 0 || (tt == 0 && thr == 0.f) || (tt == 1 && thr <= 0.f);
+    if (RB == 1 && full && !a.bimg && add == 0.f && thr_noop && scale > 0.f) {
+      static_for<0, RB>([&](auto ibb) {
+        constexpr int ib = decltype(ibb)::value;
+        float rs = 0.f, rx = 0.f;
+        static_for<0, NIMG>([&](auto jbb) {
+          constexpr int jb = decltype(jbb)::value;
+          rs += acc[ib][jb];
+          rx = fmaf((float)jb, acc[ib][jb], rx);
+        });
+        sw += rs;
+        sxw += rx;
+        syw = fmaf((float)det_row(ib), rs, syw);   // the lane's row shares one detector row index
+      });
+    } else {
+      // per detector row: pixel values (the threshold mode by uniform selects), the image row
+      // stored under one owner test, CoG sums
+      const bool store = a.bimg && owner;
+      float *bout = a.bimg + (long)frame * a.bimg_stride + pix0;
+      static_for<0, RB>([&](auto ibb) {
+        constexpr int ib = decltype(ibb)::value;
+        const int I = det_row(ib);
+        static_for<0, NIMG>([&](auto jbb) {
+          float &v = acc[ib][decltype(jbb)::value];
+          v = fmaf(v, scale, 0.f) + add;
+          const float v0 = fmaxf(v - thr, 0.f);   // type 0: (v < thr ? 0 : v - thr)
+          const float v1 = v < thr ? 0.f : v;     // type 1
+          v = tt == 0 ? v0 : (tt == 1 ? v1 : v);
+        });
+        if (store) {
+          static_for<0, NIMG>([&](auto jbb) { bout[I * NIMG + decltype(jbb)::value] = acc[ib][decltype(jbb)::value]; });
+        }
+        float rs = 0.f, rx = 0.f;
+        static_for<0, NIMG>([&](auto jbb) {
+          constexpr int jb = decltype(jbb)::value;
+          const float v = (full || (jb >= st0 && jb < en0)) ? acc[ib][jb] : 0.f;
+          rs += v;
+          rx = fmaf((float)jb, v, rx);
+        });
+        if (full || (I >= st0 && I < en0)) {
+          sw += rs;
+          sxw += rx;
+          syw = fmaf((float)I, rs, syw);
         }
       });
-    });
+    }
+    if (!owner) sw = sxw = syw = 0.f;
   } else {
     // general detector: stage the lane's pixels in its LDS slot and walk them with a
     // runtime loop, so that the rarely used per-pixel code is emitted once
