@@ -76,13 +76,13 @@ __device__ __forceinline__ ItemRef flat_item(const FusedArgs &a, long first, int
 template <int NP, int F, int B, int WAVES, bool FAST, bool SPOT>
 __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(NP <= 16 ? 2 : 1)))
 void batch_kernel(BatchArgs b, int chunk) {
-  constexpr int SPW = 64 / NP, SUB = stage_per_subap<NP, F, SPOT>();
-  __shared__ cf stage[WAVES][SPW * SUB];
+  constexpr int SPW = 64 / NP;
+  __shared__ cf stage[WAVES][wave_lds_cf<NP, F, B, SPOT>()];
   const int lane = threadIdx.x & 63;
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int sl = lane / NP;
   const int j = lane - sl * NP;
-  cf *st = &stage[w][sl * SUB];
+  cf *wl = &stage[w][0];
   WaveQueue q{b.counters, (long)gridDim.x * WAVES, chunk};
   long g = q.first((long)blockIdx.x * WAVES + w);
   if (g < b.total_groups) {
@@ -101,7 +101,7 @@ void batch_kernel(BatchArgs b, int chunk) {
         itn = flat_item<SPOT>(desc_args(b, dn), (gn - b.desc[dn].group0) * SPW, sl);
         load_item<NP>(desc_args(b, dn), itn, j, nxt);
       }
-      fused_item<NP, F, B, FAST, SPOT>(desc_args(b, d), it, j, w, sl, cur, st);
+      fused_item<NP, F, B, FAST, SPOT>(desc_args(b, d), it, j, w, sl, cur, wl);
       if (!more) break;
       cur = nxt;
       it = itn;

@@ -30,8 +30,8 @@ void fused_cog_kernel(FusedArgs a_) {
   typedef const __attribute__((address_space(4))) FusedArgs KernArgs;
   const FusedArgs &a = *(const FusedArgs *)(KernArgs *)__builtin_amdgcn_kernarg_segment_ptr();
   using S = FusedShape<NP, F, B>;
-  constexpr int SPW = S::SPW, SUB = stage_per_subap<NP, F, SPOT>();
-  __shared__ cf stage[S::WAVES][SPW * SUB];
+  constexpr int SPW = S::SPW;
+  __shared__ cf stage[S::WAVES][wave_lds_cf<NP, F, B, SPOT>()];
   const int lane = threadIdx.x & 63;
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int sl = lane / NP;
@@ -53,7 +53,7 @@ void fused_cog_kernel(FusedArgs a_) {
   auto item_of = [&](long qq, int f, int g) -> ItemRef {   // wave-uniform arguments
     return item_at(a, qq < ngx ? f : -1, (int)((sg0 + g) * SPW) + sl);
   };
-  cf *st = &stage[w][sl * SUB];
+  cf *wl = &stage[w][0];
   PhaseCol<NP> cur;
   ItemRef it = item_of(q, fcur, gcur);
   load_item<NP, true>(a, it, j, cur, fcur);
@@ -67,7 +67,7 @@ void fused_cog_kernel(FusedArgs a_) {
     const bool more = q + stride < ngx;
     const ItemRef itn = item_of(q + stride, nf, ng);
     load_item<NP, true>(a, itn, j, nxt, more ? nf : 0);   // past the end: item 0, unused
-    fused_item<NP, F, B, FAST, SPOT>(a, it, j, w, sl, cur, st);
+    fused_item<NP, F, B, FAST, SPOT>(a, it, j, w, sl, cur, wl);
     cur = nxt;
     it = itn;
     fcur = nf;

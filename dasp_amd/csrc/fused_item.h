@@ -25,6 +25,8 @@ struct FusedShape {
   static constexpr int G = B >= R ? B / R : 1;    // lanes that share one detector row
   static constexpr int TS = NP + 1;               // staged row stride (complex)
   static constexpr int WAVES = 4;
+  static constexpr int PX = RB * NIMG;            // detector pixels per lane
+  static constexpr int PXS = PX + 1;              // their LDS stride per lane (odd: no bank conflicts)
   static_assert(R * NP == F && NIMG * B == F, "shape");
   static_assert((R % B == 0) || (B % R == 0), "binning must align with rows per lane");
 };
@@ -139,12 +141,23 @@ __device__ __forceinline__ void load_item(const FusedArgs &a, const ItemRef &it,
   }
 }
 
-// LDS complex values per subaperture: the pass-1 transpose (NP rows of NP + 1), or with a spot
-// kernel the F x (NP + 1) spectrum transpose plus one value so consecutive subapertures start
-// on different banks.
+// LDS of one wave of the fused kernels, in complex values. During the transforms the region holds
+// SPW subaperture slots of fft_slot_cf() values: the pass-1 transpose (NP rows of NP + 1), or with
+// a spot kernel the F x (NP + 1) spectrum transpose plus one value so consecutive subapertures
+// start on different banks. The general detector then reuses the region for the 64 lanes'
+// pixels (PXS floats each; up to 64 pixels per lane when nimg == fftsize, more than one slot).
 template <int NP, int F, bool SPOT>
-constexpr int stage_per_subap() {
+constexpr int fft_slot_cf() {
   return SPOT ? F * (NP + 1) + 1 : NP * (NP + 1);
+}
+template <int NP, int F, int B>
+constexpr int detector_cf() {
+  return (64 * FusedShape<NP, F, B>::PXS + 1) / 2;
+}
+template <int NP, int F, int B, bool SPOT>
+constexpr int wave_lds_cf() {
+  constexpr int t = (64 / NP) * fft_slot_cf<NP, F, SPOT>(), d = detector_cf<NP, F, B>();
+  return t > d ? t : d;
 }
 
 // LGS spot convolution, doConvolution (cmod/src/centmodule.c:524-564): the reference convolves
@@ -235,11 +248,13 @@ __device__ __forceinline__ void spot_convolve(const FusedArgs &a, int s, int j, 
 }
 
 // One work item (frame, subaperture) per NP-lane group: image formation, detector, CoG.
+// wl: the wave's LDS region (wave_lds_cf values).
 template <int NP, int F, int B, bool FAST, bool SPOT = false>
 __device__ __forceinline__ void fused_item(const FusedArgs &a, const ItemRef &it, int j, int w, int sl,
-                                           const PhaseCol<NP> &col, cf *st) {
+                                           const PhaseCol<NP> &col, cf *wl) {
   using S = FusedShape<NP, F, B>;
-  constexpr int R = S::R, NIMG = S::NIMG, RB = S::RB, G = S::G, TS = S::TS;
+  constexpr int R = S::R, NIMG = S::NIMG, RB = S::RB, G = S::G, TS = S::TS, PX = S::PX, PXS = S::PXS;
+  cf *st = wl + sl * fft_slot_cf<NP, F, SPOT>();   // this lane group's subaperture slot
   const bool valid = it.valid;
   const int frame = it.frame, s = it.s;
   const int n = a.n;
@@ -363,9 +378,9 @@ __device__ __forceinline__ void fused_item(const FusedArgs &a, const ItemRef &it
     }
     if (!owner) sw = sxw = syw = 0.f;
   } else {
-    // general detector: stage the lane's pixels in its LDS slot and walk them with a
-    // runtime loop, so that the rarely used per-pixel code is emitted once
-    float *px = reinterpret_cast<float *>(st) + j * (RB * NIMG);
+    // general detector: stage the lanes' pixels in the wave's LDS region and walk them with
+    // runtime loops, so that the rarely used per-pixel code is emitted once
+    float *px = reinterpret_cast<float *>(wl) + (sl * NP + j) * PXS;
     static_for<0, RB>([&](auto ib) {
       static_for<0, NIMG>([&](auto jb) {
         px[decltype(ib)::value * NIMG + decltype(jb)::value] = acc[decltype(ib)::value][decltype(jb)::value];
@@ -384,24 +399,23 @@ __device__ __forceinline__ void fused_item(const FusedArgs &a, const ItemRef &it
     // four pixels per trip: their pre-drawn normals are loaded together (one memory latency per
     // four pixels instead of one per pixel); the pointers are null only when unused
     const bool havez = a.zpois != nullptr, haver = zread != nullptr;
-    static_assert((RB * NIMG) % 4 == 0, "pixel count per lane is a multiple of 4");
+    auto pixel = [&](int q) { return det_row(q / NIMG) * NIMG + q % NIMG; };
+    static_assert(PX % 4 == 0, "pixel count per lane is a multiple of 4");
 #pragma unroll 1
-    for (int q0 = 0; q0 < RB * NIMG; q0 += 4) {
+    for (int q0 = 0; q0 < PX; q0 += 4) {
       float zp4[4], zr4[4];
 #pragma unroll
       for (int t = 0; t < 4; ++t) {
-        const int q = q0 + t;
-        const int p = det_row(q / NIMG) * NIMG + q % NIMG;
+        const int p = pixel(q0 + t);
         zp4[t] = havez ? zpois[p] : 0.f;
         zr4[t] = haver ? zread[p] : 0.f;
       }
 #pragma unroll
       for (int t = 0; t < 4; ++t) {
         const int q = q0 + t;
-        const int I = det_row(q / NIMG), jb = q % NIMG;
         float v = px[q] * scale;   // scale is 1 where the theoretical total is not positive
         if (owner) {
-          const int p = I * NIMG + jb;
+          const int p = pixel(q);
           if (skyadd > 0.f) v += skyadd;
           if (skyarr) v += skyarr[p];
           if (pois) v = poisson_from_normal(v, zp4[t]);
@@ -419,7 +433,7 @@ __device__ __forceinline__ void fused_item(const FusedArgs &a, const ItemRef &it
     float *bout = a.bimg ? a.bimg + (long)frame * a.bimg_stride + pix0 : nullptr;
     const float *wt = a.centWeight ? a.centWeight + (a.centWeightDim == 4 ? pix0 : 0) : nullptr;
 #pragma unroll 1
-    for (int q = 0; q < RB * NIMG; ++q) {
+    for (int q = 0; q < PX; ++q) {
       const int I = det_row(q / NIMG), jb = q % NIMG;
       float v = px[q];
       if (threshType == 1) v = v < thr ? 0.f : v;
@@ -436,7 +450,7 @@ __device__ __forceinline__ void fused_item(const FusedArgs &a, const ItemRef &it
         }
       }
     }
-    wave_lds_sync();   // the slot is the next item's transpose buffer
+    wave_lds_sync();   // the region holds the next item's transposes
   }
   sw = group_sum<NP>(sw);
   sxw = group_sum<NP>(sxw);

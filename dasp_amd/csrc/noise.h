@@ -24,36 +24,38 @@ static __constant__ const float kPoisInv[40] = {
     1.0f / 25, 1.0f / 26, 1.0f / 27, 1.0f / 28, 1.0f / 29, 1.0f / 30, 1.0f / 31, 1.0f / 32,
     1.0f / 33, 1.0f / 34, 1.0f / 35, 1.0f / 36, 1.0f / 37, 1.0f / 38, 1.0f / 39, 1.0f / 40};
 
-__device__ inline float poisson_from_normal(float lam, float z) {
-  if (!(lam > 0.f)) return 0.f;
-  if (lam < 10.f) {
-    const float u = 0.5f * erfcf(-z * 0.70710678118654752f);
-    float p = expf(-lam);
-    float F = p;
-    // k = the number of leading steps i (from 1) at which u > F(i-1) and the sum still grows,
-    // F(i) > F(i-1). Both conditions are monotone: once u <= F it stays so (F never decreases),
-    // and the sum can stop growing only past the mode, where the terms keep shrinking. So k is
-    // counted without branches: a lane's flag drops at its first failing step and freezes its
-    // count, the step count is wave-uniform (1/(k+1) comes from the table through the scalar
-    // cache) and the wave leaves the loop, checked every four steps, once no lane is searching.
-    // The arithmetic of each step is that of the plain search: the same k, bit for bit.
-    float k = 0.f;
-    bool go = u > F;
-    for (int i = 0; i < 40; i += 4) {
+// 0 < lambda < 10: the inverse CDF of u = Phi(z), searched from k = 0.
+__device__ inline float poisson_small(float lam, float z) {
+  const float u = 0.5f * erfcf(-z * 0.70710678118654752f);
+  float p = expf(-lam);
+  float F = p;
+  // k = the number of leading steps i (from 1) at which u > F(i-1) and the sum still grows,
+  // F(i) > F(i-1). Both conditions are monotone: once u <= F it stays so (F never decreases),
+  // and the sum can stop growing only past the mode, where the terms keep shrinking. So k is
+  // counted without branches: a lane's flag drops at its first failing step and freezes its
+  // count, the step count is wave-uniform (1/(k+1) comes from the table through the scalar
+  // cache) and the wave leaves the loop, checked every four steps, once no lane is searching.
+  // The arithmetic of each step is that of the plain search: the same k, bit for bit.
+  float k = 0.f;
+  bool go = u > F;
+  for (int i = 0; i < 40; i += 4) {
 #pragma unroll
-      for (int t = 0; t < 4; ++t) {
-        const float pn = p * (lam * kPoisInv[i + t]);
-        const float Fn = F + pn;
-        go = go && (Fn > F);
-        k += go ? 1.f : 0.f;
-        go = go && (u > Fn);
-        p = pn;
-        F = Fn;
-      }
-      if (!__any(go)) break;
+    for (int t = 0; t < 4; ++t) {
+      const float pn = p * (lam * kPoisInv[i + t]);
+      const float Fn = F + pn;
+      go = go && (Fn > F);
+      k += go ? 1.f : 0.f;
+      go = go && (u > Fn);
+      p = pn;
+      F = Fn;
     }
-    return k;
+    if (!__any(go)) break;
   }
+  return k;
+}
+
+// lambda >= 10: the Cornish-Fisher quantile.
+__device__ inline float poisson_large(float lam, float z) {
   const float s = sqrtf(lam + 1.0f / 12.0f);
   const float s2 = s * s;
   const float g1 = lam / (s2 * s);
@@ -63,6 +65,12 @@ __device__ inline float poisson_from_normal(float lam, float z) {
                   g1 * g1 * (2.0f * z2 * z - 5.0f * z) / 36.0f;
   const float k = ceilf(lam + s * w - 0.5f);
   return k > 0.f ? k : 0.f;
+}
+
+__device__ inline float poisson_from_normal(float lam, float z) {
+  if (!(lam > 0.f)) return 0.f;
+  if (lam < 10.f) return poisson_small(lam, z);
+  return poisson_large(lam, z);
 }
 
 }  // namespace dasp

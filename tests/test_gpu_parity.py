@@ -65,8 +65,15 @@ def test_noise_poisson_read_relative_floor(path):
     c = Case(w.nsubx, w.n, w.nfft, w.nimg, obs=w.obs, sig=w.sig, readnoise=w.readnoise, addPoisson=1,
              noiseFloor=w.noiseFloor, threshType=0, relativeFloor=True, phs=w.phase(), seed=5, form=PATHS[path])
     oc, ob, _ = c.run_both()
+    _check_noisy(c, oc, ob)
+    c.free()
+
+
+def _check_noisy(c, oc, ob):
+    """Noisy images against the oracle: a few pixels may round to the neighbouring count (FP32
+    against FP64 at a step of the inverse CDF); the other sub-apertures must match."""
     act = _active(c)
-    b = c.bimg.reshape(-1, w.nimg, w.nimg)[act]
+    b = c.bimg.reshape(-1, c.nimg, c.nimg)[act]
     diffpix = np.abs(b - ob[act]) > 1e-3 * np.maximum(1, np.abs(ob[act]))
     say("noisy pixels differing: %d of %d" % (diffpix.sum(), diffpix.size))
     assert diffpix.sum() <= max(3, 1e-4 * diffpix.size)
@@ -76,6 +83,42 @@ def test_noise_poisson_read_relative_floor(path):
     assert d[clean].max() < 1e-4
     # the noisy images are integers plus read noise: check the photon count statistics
     assert abs(b.sum() / act.sum() - ob[act].sum() / act.sum()) < 5
+
+
+FUSED_SHAPES = {
+    # name: (nsubx, n, nfft, nimg), fused kernel <NP, F, F/nimg>; detector pixels per lane
+    "n8_f16_b1": (8, 8, 16, 16),      # 32 per lane (more than one subaperture slot of LDS)
+    "n16_f32_b1": (6, 16, 32, 32),    # 64 per lane (more than one subaperture slot of LDS)
+    "n8_f16_b2": (8, 8, 16, 8),       # 8 per lane
+    "n16_f32_b4": (6, 16, 32, 8),     # 8 per lane, two lanes per detector row
+    "n32_f64_b4": (6, 32, 64, 16),    # 16 per lane
+    "n32_f64_b2": (4, 32, 64, 32),    # 32 per lane
+}
+
+
+@pytest.mark.parametrize("kind", ["noise", "skyarr"])
+@pytest.mark.parametrize("name", sorted(FUSED_SHAPES))
+def test_general_detector_every_fused_shape(name, kind):
+    """The fused kernels' general detector on every shape: it stages all the lanes' pixels in the
+    wave's LDS region, up to 64 per lane when nimg == fftsize (more than a subaperture's transpose
+    slot holds). Poisson + read noise with a relative floor (against the oracle with the flipped-
+    pixel bounds), or a sky-background array without noise (exact)."""
+    nsubx, n, nfft, nimg = FUSED_SHAPES[name]
+    say("case general detector %s %s" % (name, kind))
+    w = W.WFS(name, nsubx, n, nfft, nimg, 4.2, 0.2, 0.15, 25.0)
+    if kind == "noise":
+        c = Case(nsubx, n, nfft, nimg, obs=0.2, sig=500.0, readnoise=0.5, addPoisson=1, noiseFloor=0.05,
+                 threshType=0, relativeFloor=True, phs=w.phase(), seed=3)
+        oc, ob, _ = c.run_both()
+        _check_noisy(c, oc, ob)
+    else:
+        sky = (np.random.default_rng(2).random((nsubx * nsubx, nimg, nimg)) * 0.5).astype(np.float32)
+        c = Case(nsubx, n, nfft, nimg, obs=0.2, sig=200.0, skybrightness=sky, phs=w.phase())
+        oc, ob, _ = c.run_both()
+        act = _active(c)
+        b = c.bimg.reshape(-1, nimg, nimg)[act]
+        np.testing.assert_allclose(b, ob[act], rtol=0, atol=2e-5 * ob[act].max())
+        assert report("skyarr " + name, c.cents, oc, act).max() < 1e-4
     c.free()
 
 
