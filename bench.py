@@ -21,6 +21,9 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 MFMA_PEAK_TFLOPS = {"bf16": 2500.0, "fp16": 2500.0}
+# Vector issue peak: one wave64 VALU instruction per CU per cycle, 256 CUs at the 2.4 GHz peak
+# engine clock (FP32 work here issues no packed FP32 instructions, so this is also the FP32 roof).
+VALU_PEAK_GINST_S = 256 * 2.4
 
 
 def algorithmic_bytes(w, nactive, frames, images, noise):
@@ -48,6 +51,31 @@ def measured_traffic(workload, form):
     return None
 
 
+def measured_valu(workload, form):
+    """Wave-level VALU instructions per subaperture image of the main kernel (rocprofv3
+    SQ_INSTS_VALU / images, lines 'valu: <workload> <form> <count>' under profiles/), or None."""
+    path = os.path.join(ROOT, "profiles", "valu_counts.md")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        for line in f:
+            p = line.split()
+            if len(p) == 4 and p[0] == "valu:" and p[1] == workload and p[2] == form:
+                return float(p[3])
+    return None
+
+
+def valu_roof(workload, form, images, kavg):
+    """Vector-issue roofline: the counted VALU instructions per image at the measured rate against
+    VALU_PEAK_GINST_S (the roof that binds the FP32 kernels)."""
+    per = measured_valu(workload, form)
+    if per is None:
+        return None
+    ach = per * images / kavg / 1e9
+    return {"insts_per_image": per, "achieved": ach, "peak": VALU_PEAK_GINST_S, "unit": "G wave-instr/s",
+            "frac": ach / VALU_PEAK_GINST_S, "source": "profiles/valu_counts.md (SQ_INSTS_VALU per image)"}
+
+
 def dft_mfma_per_image(w):
     return 18 if w.nfft == 32 else 144
 
@@ -62,24 +90,60 @@ def roofline(form, w, nact, frames, kavg, abytes, noise):
     ach = abytes / kavg / 1e9
     per = measured_traffic(w.name, form)
     traffic = per * nact * frames if per is not None and not noise else None
+    tsrc = "profiles/hbm_traffic.md (rocprofv3 FETCH_SIZE/WRITE_SIZE per image, scaled)" if traffic else None
+    valu = valu_roof(w.name, form, nact * frames, kavg)
     if form == "fft32":
         return {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
-                "traffic": traffic, "kernel_avg_ms": kavg * 1e3, "algorithmic_bytes": abytes}
+                "traffic": traffic, "traffic_source": tsrc, "kernel_avg_ms": kavg * 1e3,
+                "algorithmic_bytes": abytes, "valu": valu}
     # DFT-GEMM form, v_mfma_f32_32x32x16 per subaperture image (hi/lo split, 3 products):
     # 18 for n = 16 / nfft = 32, 144 for n <= 32 / nfft = 64
     flops = dft_mfma_per_image(w) * 2 * 32 * 32 * 16 * nact * frames
     tf = flops / kavg / 1e12
     return {"bound": "mfma", "achieved": tf, "peak": MFMA_PEAK_TFLOPS[form], "unit": "TFLOP/s",
-            "frac": tf / MFMA_PEAK_TFLOPS[form], "traffic": traffic, "kernel_avg_ms": kavg * 1e3,
+            "frac": tf / MFMA_PEAK_TFLOPS[form], "traffic": traffic, "traffic_source": tsrc,
+            "kernel_avg_ms": kavg * 1e3,
             "mfma_flops": flops, "useful_flops": 8 * (w.nfft * w.n ** 2 + w.nfft ** 2 * w.n) * nact * frames,
-            "hbm_achieved_gbs": ach, "algorithmic_bytes": abytes}
+            "hbm_achieved_gbs": ach, "algorithmic_bytes": abytes, "valu": valu}
+
+
+def host_cpus():
+    """The host CPUs this process may use and how that number was found: the scheduler affinity
+    mask, bounded by the cgroup CPU quota and by OMP_NUM_THREADS when the environment sets one
+    (the share of the host this process is given). Returns (threads, description dict)."""
+    aff = len(os.sched_getaffinity(0))
+    n = aff
+    quota = None
+    try:
+        with open("/sys/fs/cgroup/cpu.max") as f:
+            q, per = f.read().split()[:2]
+        if q != "max":
+            quota = int(q) / int(per)
+            n = min(n, max(1, int(quota)))
+    except (OSError, ValueError):
+        pass
+    omp = os.environ.get("OMP_NUM_THREADS")
+    if omp and omp.isdigit() and int(omp) > 0:
+        n = min(n, int(omp))
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return n, {"nproc": os.cpu_count(), "affinity": aff, "cgroup_quota": quota, "omp_num_threads": omp,
+               "model": model}
 
 
 def cpu_baseline(w, phs0, budget_s=10.0, corr=None):
     """CPU leg: the FP32 C restatement of the reference engine (oracle/cref), threaded like the
-    reference's pthread pool over this host's cores (at most 16), timed on repeated frames of the
-    same workload until the budget is spent; then the FP64 oracle on frame 0 for the error figure.
-    Returns (subap-images/s, threads, sample description, FP64 slopes of frame 0)."""
+    reference's pthread pool over every host CPU this process is given (host_cpus), timed on
+    repeated frames of the same workload until the budget is spent; then the FP64 oracle on
+    frame 0 for the error figure.
+    Returns (subap-images/s, threads, sample description, FP64 slopes of frame 0, host dict)."""
     from oracle import centoracle as O
     from oracle.cref import CRef
     from dasp_amd.util.tel import obstructed_pupil
@@ -88,7 +152,7 @@ def cpu_baseline(w, phs0, budget_s=10.0, corr=None):
     flag = pup.getSubapFlag(w.nsubx, 0.5)
     frac = (pup.getSubarea(w.nsubx) / (w.n * w.n)).astype(np.float32)
     nact = int(flag.sum())
-    nthr = max(1, min(16, os.cpu_count() or 1))
+    nthr, host = host_cpus()
     cr = CRef(w.nsubx, w.n, w.nfft, w.clip, w.nimg, w.cen, pupfn.reshape(-1, w.n, w.n), flag, frac, sig=float(w.sig),
               corrPattern=corr, parabolicFit=w.parabolicFit if corr is not None else 0)
     cr.run(phs0, nthreads=nthr)
@@ -106,7 +170,7 @@ def cpu_baseline(w, phs0, budget_s=10.0, corr=None):
     oc, _, _ = O.run_phase(p, phs0)
     sample = ("%d frames x %d active subapertures of %s, FP32 C restatement of centmodule.c (own radix-2 FFT, "
               "no FFTW), %d threads, %.1f s" % (frames, nact, w.name, nthr, dt))
-    return nact * frames / dt, nthr, sample, oc
+    return nact * frames / dt, nthr, sample, oc, host
 
 
 def main():
@@ -191,9 +255,10 @@ def main():
         wb.phs[0].copy_(phs)
         wb.tune(reps=3)
 
-    def run_form(form):
+    def run_form(form, fast_trig=not args.accurate_trig):
         """Time args.steps steps of one compute form; returns (ms per step, kernel avg s, slopes of frame 0)."""
         cent.setForm(h, {"fft32": 0, "bf16": 1, "fp16": 2}[form])
+        cent.setFastTrig(h, fast_trig)
         cents = wb.cents[0].view(frames, S, 2) if wb is not None else d_cents
 
         def step():
@@ -267,9 +332,9 @@ def main():
     if not args.no_cpu:
         # CPU leg: the C restatement timed on this host; the FP64 oracle on frame 0 gives the
         # error figure of every form.
-        v, nthr, sample, oc = cpu_baseline(w, w.phase(0), corr=corr)
+        v, nthr, sample, oc, host = cpu_baseline(w, w.phase(0), corr=corr)
         result["cpu_baseline"] = {"value": v, "unit": "subap-images/s", "cores": nthr, "kind": "port",
-                                  "sample": sample}
+                                  "sample": sample, "host": host}
     act = cc.subflag.ravel() == 1
     if oc is not None and not noise:
         d = (c0 - oc.reshape(-1, 2))[act]
@@ -290,7 +355,21 @@ def main():
                 e["rms_err_vs_fp64_px"] = float(np.sqrt((d ** 2).mean()))
                 e["max_err_vs_fp64_px"] = float(np.abs(d).max())
             forms[form] = e
+        # the plain form with the other trig mode (sincosf when the headline uses the reduced
+        # hardware sin/cos, and the reverse)
+        other = "fft32_sincosf" if not args.accurate_trig else "fft32_fast_trig"
+        fms, fk, fc = run_form("fft32", fast_trig=args.accurate_trig)
+        e = {"value": nact * frames / (fms / 1e3), "ms_per_step": fms, "fast_trig": args.accurate_trig,
+             "roofline": roofline("fft32", w, nact, frames, fk, abytes, noise) if fk else None}
+        if e["roofline"] is not None:
+            e["roofline"]["valu"] = None     # the counted instructions are the headline trig mode's
+        if oc is not None:
+            d = (fc - oc)[act]
+            e["rms_err_vs_fp64_px"] = float(np.sqrt((d ** 2).mean()))
+            e["max_err_vs_fp64_px"] = float(np.abs(d).max())
+        forms[other] = e
         cent.setForm(h, {"fft32": 0, "bf16": 1, "fp16": 2}[args.form])
+        cent.setFastTrig(h, not args.accurate_trig)
         result["forms"] = forms
     if wb is not None:
         wb.free()

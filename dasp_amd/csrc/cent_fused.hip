@@ -112,6 +112,9 @@ bool fused_supported(int np, int F, int nimg, bool spot) {
 }
 
 hipError_t launch_fused(const FusedArgs &a, int np, int F, bool spot, hipStream_t st) {
+  // the argument block comes from another translation unit: refuse, rather than divide by, a
+  // detector size that cannot be right
+  if (a.nimg <= 0 || F % a.nimg != 0) return hipErrorInvalidValue;
   const int b = F / a.nimg;
 #define DASP_FUSED_CASE(A, Bf, C) \
   if (np == A && F == Bf && b == C) return launch_fused_t<A, Bf, C, false>(a, st);
