@@ -20,7 +20,11 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-MFMA_PEAK_TFLOPS = {"bf16": 2500.0, "fp16": 2500.0}
+MFMA_PEAK_TFLOPS = {"bf16": 2500.0, "fp16": 2500.0, "fp16x1": 2500.0}
+# engine form codes (DASP_CENT_FORM_*): the compensated DFT forms (three products of hi/lo
+# splits) and the single-product FP16 one (x1)
+FORM_CODE = {"fft32": 0, "bf16": 1, "fp16": 2, "fp16x1": 4}
+FORM_DTYPE = {"fft32": "f32", "bf16": "bf16x3+f32acc", "fp16": "f16x3+f32acc", "fp16x1": "f16x1+f32acc"}
 # Vector issue peak: one wave64 VALU instruction per CU per cycle, 256 CUs at the 2.4 GHz peak
 # engine clock (FP32 work here issues no packed FP32 instructions, so this is also the FP32 roof).
 VALU_PEAK_GINST_S = 256 * 2.4
@@ -76,8 +80,9 @@ def valu_roof(workload, form, images, kavg):
             "frac": ach / VALU_PEAK_GINST_S, "source": "profiles/valu_counts.md (SQ_INSTS_VALU per image)"}
 
 
-def dft_mfma_per_image(w):
-    return 18 if w.nfft == 32 else 144
+def dft_mfma_per_image(w, form):
+    per = 18 if w.nfft == 32 else 144
+    return per // 3 if form.endswith("x1") else per
 
 
 def dft_form_shape(w):
@@ -97,8 +102,8 @@ def roofline(form, w, nact, frames, kavg, abytes, noise):
                 "traffic": traffic, "traffic_source": tsrc, "kernel_avg_ms": kavg * 1e3,
                 "algorithmic_bytes": abytes, "valu": valu}
     # DFT-GEMM form, v_mfma_f32_32x32x16 per subaperture image (hi/lo split, 3 products):
-    # 18 for n = 16 / nfft = 32, 144 for n <= 32 / nfft = 64
-    flops = dft_mfma_per_image(w) * 2 * 32 * 32 * 16 * nact * frames
+    # 18 for n = 16 / nfft = 32, 144 for n <= 32 / nfft = 64; a third of that with one product
+    flops = dft_mfma_per_image(w, form) * 2 * 32 * 32 * 16 * nact * frames
     tf = flops / kavg / 1e12
     return {"bound": "mfma", "achieved": tf, "peak": MFMA_PEAK_TFLOPS[form], "unit": "TFLOP/s",
             "frac": tf / MFMA_PEAK_TFLOPS[form], "traffic": traffic, "traffic_source": tsrc,
@@ -178,7 +183,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--form", default="fft32", choices=["fft32", "bf16", "fp16"])
+    ap.add_argument("--form", default="fft32", choices=sorted(FORM_CODE))
     workloads = {"scao8x8": "SCAO8", "wfs40x40_noise": "NOISE40", "wfs80x80_nfft32": "WFS80",
                  "wfs80x80_nyquist_corr": "CORR80"}   # name -> dasp_amd.workloads attribute
     ap.add_argument("--workload", default="wfs80x80_nfft32", choices=sorted(workloads) + ["multi_wfs"])
@@ -228,7 +233,7 @@ def main():
     h = cc.centstruct
     if w.relativeFloor:
         cent.setRelativeFloor(h, True)
-    cent.setForm(h, {"fft32": 0, "bf16": 1, "fp16": 2}[args.form])
+    cent.setForm(h, FORM_CODE[args.form])
     cent.setFastTrig(h, not args.accurate_trig)
     nact, npup, csz = cent.info(h)
 
@@ -257,7 +262,7 @@ def main():
 
     def run_form(form, fast_trig=not args.accurate_trig):
         """Time args.steps steps of one compute form; returns (ms per step, kernel avg s, slopes of frame 0)."""
-        cent.setForm(h, {"fft32": 0, "bf16": 1, "fp16": 2}[form])
+        cent.setForm(h, FORM_CODE[form])
         cent.setFastTrig(h, fast_trig)
         cents = wb.cents[0].view(frames, S, 2) if wb is not None else d_cents
 
@@ -318,7 +323,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": {"fft32": "f32", "bf16": "bf16x3+f32acc", "fp16": "f16x3+f32acc"}[args.form],
+        "dtype": FORM_DTYPE[args.form],
         "data": "synthetic",
         "config": {"workload": w.name, "nsubx": w.nsubx, "n": w.n, "nfft": w.nfft, "nimg": w.nimg,
                    "npup": w.npup, "frames_per_step": frames, "active_subaps": nact, "form": args.form,
@@ -343,7 +348,7 @@ def main():
     if args.compare_forms and wb is None and not noise and dft_form_shape(w):
         # the headline compares the plain FP32 form with the MFMA DFT-GEMM forms on the same input
         forms = {}
-        for form in ("fft32", "fp16", "bf16"):
+        for form in ("fft32", "fp16", "bf16", "fp16x1"):
             if form == args.form:
                 fms, fk, fc = ms_step, kavg, c0
             else:
@@ -368,7 +373,9 @@ def main():
             e["rms_err_vs_fp64_px"] = float(np.sqrt((d ** 2).mean()))
             e["max_err_vs_fp64_px"] = float(np.abs(d).max())
         forms[other] = e
-        cent.setForm(h, {"fft32": 0, "bf16": 1, "fp16": 2}[args.form])
+        for k, e in forms.items():   # step-time ratio of the plain FP32 form to each form
+            e["speedup_vs_fft32"] = forms["fft32"]["ms_per_step"] / e["ms_per_step"]
+        cent.setForm(h, FORM_CODE[args.form])
         cent.setFastTrig(h, not args.accurate_trig)
         result["forms"] = forms
     if wb is not None:

@@ -503,9 +503,12 @@ static bool fused_options(dasp_cent *c) {
 
 // The DFT-GEMM kernels cover the fused options for n = 16 / fftsize 32 and n in 17..32 /
 // fftsize 64, nimg 16.
+static bool is_dft_form(int form) {
+  return form == DASP_CENT_FORM_DFT_FP16 || form == DASP_CENT_FORM_DFT_BF16 || form == DASP_CENT_FORM_DFT_FP16_1;
+}
+
 static bool use_dft(dasp_cent *c) {
-  return (c->form == DASP_CENT_FORM_DFT_FP16 || c->form == DASP_CENT_FORM_DFT_BF16) && fused_options(c) &&
-         c->spotDim == 0 && dft_supported(c->n, c->F, c->nimg);
+  return is_dft_form(c->form) && fused_options(c) && c->spotDim == 0 && dft_supported(c->n, c->F, c->nimg);
 }
 
 static bool use_fused(dasp_cent *c) {
@@ -604,7 +607,8 @@ static int launch_phase(dasp_cent *c, const float *d_phs, long phs_stride, int n
     }
     int trc = timing_mark(c, st);
     if (trc) return trc;
-    if (use_dft(c)) HIPCHK(launch_dft(fa, c->form == DASP_CENT_FORM_DFT_BF16, st));
+    if (use_dft(c))
+      HIPCHK(launch_dft(fa, c->form == DASP_CENT_FORM_DFT_BF16, c->form != DASP_CENT_FORM_DFT_FP16_1, st));
     else HIPCHK(launch_fused(fa, np, c->F, c->spotDim != 0, st));
     trc = timing_mark(c, st);   // the timed interval is the image kernel alone
     if (trc) return trc;
@@ -1265,8 +1269,8 @@ int dasp_cent_free(dasp_cent_t *c) {
 int dasp_cent_set_form(dasp_cent_t *c, int form) {
   if (!c) return fail(DASP_CENT_EINVAL, "null handle");
   c->gen++;
-  if (form < DASP_CENT_FORM_FFT32 || form > DASP_CENT_FORM_FFT32_GENERAL) return fail(DASP_CENT_EINVAL, "unknown form %d", form);
-  if ((form == DASP_CENT_FORM_DFT_FP16 || form == DASP_CENT_FORM_DFT_BF16) &&
+  if (form < DASP_CENT_FORM_FFT32 || form > DASP_CENT_FORM_DFT_FP16_1) return fail(DASP_CENT_EINVAL, "unknown form %d", form);
+  if (is_dft_form(form) &&
       !(fused_options(c) && c->spotDim == 0 && dft_supported(c->n, c->F, c->nimg)))
     return fail(DASP_CENT_EUNSUPPORTED, "the DFT-GEMM form covers n=16 with fftsize=clipsize=32 and n=17..32 with "
                                         "fftsize=clipsize=64, nimg=16, the fused options, no spot kernel, phase only");

@@ -298,18 +298,23 @@ __global__ __launch_bounds__(256) void dft16_cog_kernel(FusedArgs a_) {
 // instructions per pair but needs 171 VGPRs (two waves). Measured on the 80x80 nfft32 workload,
 // the first is 1-2 % faster for FP16 and the second 2-3 % faster for BF16 (whose split code makes
 // it a two-wave kernel either way), so each type takes its faster variant.
-template <typename T8, bool FAST, bool FULLWIN>
-constexpr bool kPipeThreeWaves = std::is_same<T8, h8>::value && FAST && FULLWIN;
+template <typename T8, bool FAST, bool FULLWIN, bool COMP>
+constexpr bool kPipeThreeWaves = COMP && std::is_same<T8, h8>::value && FAST && FULLWIN;
+// waves per SIMD asked of the compiler (1: its own choice)
+template <typename T8, bool FAST, bool FULLWIN, bool COMP>
+constexpr int kPipeWaves = !COMP ? (FAST ? 3 : 2) : (kPipeThreeWaves<T8, FAST, FULLWIN, COMP> ? 3 : 1);
 
-template <typename T8, bool FAST, bool FULLWIN>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kPipeThreeWaves<T8, FAST, FULLWIN> ? 3 : 1)))
+// COMP: the three-product scheme; without it the single-product form (one product of the FP16
+// roundings, a third of the matrix-core work and no lo parts).
+template <typename T8, bool FAST, bool FULLWIN, bool COMP>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kPipeWaves<T8, FAST, FULLWIN, COMP>)))
 void dft16_pipe_kernel(FusedArgs a_) {
   const FusedArgs &a = a_;
   const int l = threadIdx.x & 63;
   const int h = l >> 5;
   const int c = l & 31;
   const int jj = c & 15;
-  CConst<T8> B1, A2;   // as in dft16_cog_kernel, times the tilt factors
+  CConst<T8, COMP> B1, A2;   // as in dft16_cog_kernel, times the tilt factors
   {
     float re[8], im[8], re2[8], im2[8];
     const double ts = (double)a.tiltstep, twopi_f = 6.283185307179586 / kF;
@@ -390,7 +395,7 @@ void dft16_pipe_kernel(FusedArgs a_) {
       pu8[t] = *(const float *)(pu + t * (kN * 4) + uo);
     }
   };
-  auto phase_factor = [&](const float (&ph8)[8], const float (&pu8)[8], COp<T8> &A1) {
+  auto phase_factor = [&](const float (&ph8)[8], const float (&pu8)[8], COp<T8, COMP> &A1) {
     float er[8], ei[8];
 #pragma unroll
     for (int t = 0; t < 8; ++t) {
@@ -418,7 +423,7 @@ void dft16_pipe_kernel(FusedArgs a_) {
   load_pair(wave_id, cf, cp, ph8, pu8);
   f16v zr = {}, zi = {};
   {
-    COp<T8> A1;
+    COp<T8, COMP> A1;
     phase_factor(ph8, pu8, A1);
     cgemm_vc(A1, B1, zr, zi);
   }
@@ -432,7 +437,7 @@ void dft16_pipe_kernel(FusedArgs a_) {
     // of item 1, the wave sums and the loads two pairs ahead.
     const int f = cf, p = cp;
     // ---- region 1 ----
-    COp<T8> B2[2];
+    COp<T8, COMP> B2[2];
 #pragma unroll
     for (int sub = 0; sub < 2; ++sub) {
       float br[8], bi[8];
@@ -446,7 +451,7 @@ void dft16_pipe_kernel(FusedArgs a_) {
     f16v xr0 = {}, xi0 = {}, xr1 = {}, xi1 = {};
     cgemm_cv(A2, B2[0], xr0, xi0);
     cgemm_cv(A2, B2[1], xr1, xi1);
-    COp<T8> A1n;
+    COp<T8, COMP> A1n;
     phase_factor(ph8, pu8, A1n);   // pair q + nwaves (loaded one iteration ago)
     __builtin_amdgcn_sched_barrier(0);
     // ---- region 2 ----
@@ -497,7 +502,7 @@ void dft16_pipe_kernel(FusedArgs a_) {
     __builtin_amdgcn_sched_barrier(0);
     // ---- region 3 ----
     detector(1, xr1, xi1);
-    if constexpr (kPipeThreeWaves<T8, FAST, FULLWIN>) {
+    if constexpr (kPipeThreeWaves<T8, FAST, FULLWIN, COMP>) {
 #pragma unroll
       for (int sub = 0; sub < 2; ++sub)
 #pragma unroll
@@ -549,14 +554,14 @@ void dft16_pipe_kernel(FusedArgs a_) {
 // Persistent launch of the pipelined kernel: every wave builds the DFT-matrix operands once, so
 // the grid is the resident wave count (workgroups per CU from the occupancy query, times the CU
 // count) or the pair count if smaller, in whole multiples of the XCD count.
-template <typename T8, bool FAST, bool FULLWIN>
+template <typename T8, bool FAST, bool FULLWIN, bool COMP>
 hipError_t launch_pipe_t(const FusedArgs &a, long npairs, hipStream_t st) {
   static int per_cu = 0, cus = 0;
   if (per_cu == 0) {
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess ||
         hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
-        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, dft16_pipe_kernel<T8, FAST, FULLWIN>, 256, 0) !=
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, dft16_pipe_kernel<T8, FAST, FULLWIN, COMP>, 256, 0) !=
             hipSuccess || per_cu < 1 || cus < 1) {
       per_cu = 2;
       cus = 256;
@@ -566,22 +571,27 @@ hipError_t launch_pipe_t(const FusedArgs &a, long npairs, hipStream_t st) {
   const long cap = (long)cus * per_cu;
   if (blocks > cap) blocks = cap;
   blocks = (blocks + kXcds - 1) / kXcds * kXcds;   // every XCD gets the same number of workgroups
-  hipLaunchKernelGGL((dft16_pipe_kernel<T8, FAST, FULLWIN>), dim3((unsigned)blocks), dim3(256), 0, st, a);
+  hipLaunchKernelGGL((dft16_pipe_kernel<T8, FAST, FULLWIN, COMP>), dim3((unsigned)blocks), dim3(256), 0, st, a);
   return hipGetLastError();
+}
+
+template <typename T8, bool FAST, bool COMP>
+hipError_t launch_pipe_c(const FusedArgs &a, bool full, long npairs, hipStream_t st) {
+  return full ? launch_pipe_t<T8, FAST, true, COMP>(a, npairs, st) : launch_pipe_t<T8, FAST, false, COMP>(a, npairs, st);
 }
 
 template <typename T8, bool FAST>
 hipError_t launch_pipe(const FusedArgs &a, bool full, long npairs, hipStream_t st) {
-  return full ? launch_pipe_t<T8, FAST, true>(a, npairs, st) : launch_pipe_t<T8, FAST, false>(a, npairs, st);
+  return launch_pipe_c<T8, FAST, true>(a, full, npairs, st);
 }
 
 bool dft_supported(int n, int F, int nimg) {
   return (n == kN && F == kF && nimg == kM) || dft32_supported(n, F, nimg);
 }
 
-hipError_t launch_dft(const FusedArgs &a, int bf16, hipStream_t st) {
+hipError_t launch_dft(const FusedArgs &a, int bf16, bool comp, hipStream_t st) {
   if (a.nitems <= 0) return hipSuccess;
-  if (a.n != kN) return launch_dft32(a, bf16, st);
+  if (a.n != kN) return launch_dft32(a, bf16, comp, st);
   const long npairs = (a.nitems + 1) / 2 + kXcds;
   long blocks = (npairs + 3) / 4;
   // persistent grid for the general-detector kernel: every wave builds the DFT-matrix operands
@@ -595,9 +605,14 @@ hipError_t launch_dft(const FusedArgs &a, int bf16, hipStream_t st) {
       if (a.fast_trig) return launch_pipe<b8, true>(a, full, npairs, st);
       return launch_pipe<b8, false>(a, full, npairs, st);
     }
+    if (!comp) {   // the single-product form is FP16 only
+      if (a.fast_trig) return launch_pipe_c<h8, true, false>(a, full, npairs, st);
+      return launch_pipe_c<h8, false, false>(a, full, npairs, st);
+    }
     if (a.fast_trig) return launch_pipe<h8, true>(a, full, npairs, st);
     return launch_pipe<h8, false>(a, full, npairs, st);
   }
+  // the general detector options run the three-product kernel in every DFT form
   if (bf16) {
     if (a.fast_trig) hipLaunchKernelGGL((dft16_cog_kernel<b8, true>), dim3((unsigned)blocks), dim3(256), 0, st, a);
     else hipLaunchKernelGGL((dft16_cog_kernel<b8, false>), dim3((unsigned)blocks), dim3(256), 0, st, a);

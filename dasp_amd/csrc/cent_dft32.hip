@@ -12,7 +12,8 @@
 // (lt, kt) holds X[k][l] with lane = k - 32 kt + 32 h and register 4g + u <-> l = 32 lt + 8g + 4h
 // + u: a 4x4 detector pixel is four registers in-lane (u) summed over the four lanes of a quad
 // (k), and after the 180-degree map it is pixel (I, J) = (15 - 8 kt - c/4, 15 - 8 lt - 2g - h).
-// The DFT-matrix operands (eight of them) are built once per workgroup into LDS.
+// The DFT-matrix operands (eight of them) are built once per workgroup into LDS. COMP selects the
+// three-product scheme; without it one product of the FP16 roundings (the single-product form).
 #include "dft_ops.h"
 #include "noise.h"
 #include "wave_ops.h"
@@ -24,13 +25,13 @@ constexpr int kF64 = 64, kM64 = 16, kXcds = 8;
 }
 
 
-template <typename T8, bool FAST, bool FULL>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void dft32_cog_kernel(FusedArgs a_) {
+template <typename T8, bool FAST, bool FULL, bool COMP>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(COMP || !FULL ? 2 : 3))) void dft32_cog_kernel(FusedArgs a_) {
   typedef const __attribute__((address_space(4))) FusedArgs KernArgs;
   const FusedArgs &a = *(const FusedArgs *)(KernArgs *)__builtin_amdgcn_kernarg_segment_ptr();
   constexpr int F = kF64, M = kM64;
-  __shared__ CConstLds<T8> w1[2][2];   // [kt][ks]: W[32 kt + c][16 ks + 8h + t]
-  __shared__ CConstLds<T8> w2[2][2];   // [lt][ks]: W[32 lt + c][16 ks + 8(t/4) + 4h + t%4]
+  __shared__ CConstLds<T8, COMP> w1[2][2];   // [kt][ks]: W[32 kt + c][16 ks + 8h + t]
+  __shared__ CConstLds<T8, COMP> w2[2][2];   // [lt][ks]: W[32 lt + c][16 ks + 8(t/4) + 4h + t%4]
   __shared__ float pxbuf[4][64 * 4];
   const int l = threadIdx.x & 63;
   const int h = l >> 5;
@@ -50,7 +51,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void d
           const int j = 16 * ks + 8 * (t / 4) + 4 * h + t % 4;
           sincospif(-2.0f * (float)((k * j) % F) / F, &im2[t], &re2[t]);
         }
-        CConst<T8> v1, v2;
+        CConst<T8, COMP> v1, v2;
         v1.set(re, im);
         v2.set(re2, im2);
         w1[tile][ks].put(l, v1);
@@ -124,7 +125,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void d
     }
     const int frame = cf, s = scalar_read(a.active, s0 + cidx);
     // ---- E = pup exp(i (phi - t)) and its hi/lo operands ----
-    COp<T8> A1[2];
+    COp<T8, COMP> A1[2];
 #pragma unroll
     for (int ks = 0; ks < 2; ++ks) {
       float er[8], ei[8];
@@ -148,12 +149,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void d
     const bool b0 = (c & 1) != 0, b1 = (c & 2) != 0;
 #pragma unroll
     for (int kt = 0; kt < 2; ++kt) {
-      COp<T8> B2[2];   // [ks]
+      COp<T8, COMP> B2[2];   // [ks]
       {
         f16v zr = {}, zi = {};
 #pragma unroll
         for (int ks = 0; ks < 2; ++ks) {
-          const CConst<T8> w = w1[kt][ks].get(l + lofs);
+          const CConst<T8, COMP> w = w1[kt][ks].get(l + lofs);
           cgemm_vc(A1[ks], w, zr, zi);
         }
 #pragma unroll
@@ -175,7 +176,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void d
         f16v xr = {}, xi = {};
 #pragma unroll
         for (int ks = 0; ks < 2; ++ks) {
-          const CConst<T8> w = w2[lt][ks].get(l + lofs);
+          const CConst<T8, COMP> w = w2[lt][ks].get(l + lofs);
           cgemm_cv(w, B2[ks], xr, xi);
         }
         float v[4];
@@ -301,26 +302,53 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void d
 
 bool dft32_supported(int n, int F, int nimg) { return n > 16 && n <= 32 && F == kF64 && nimg == kM64; }
 
-template <typename T8, bool FAST>
-static void launch_dft32_t(const FusedArgs &a, long blocks, hipStream_t st) {
-  if (a.n == 32) hipLaunchKernelGGL((dft32_cog_kernel<T8, FAST, true>), dim3((unsigned)blocks), dim3(256), 0, st, a);
-  else hipLaunchKernelGGL((dft32_cog_kernel<T8, FAST, false>), dim3((unsigned)blocks), dim3(256), 0, st, a);
+// Resident workgroups per CU of one kernel instance (occupancy query, once per instance).
+template <typename T8, bool FAST, bool FULL, bool COMP>
+static long dft32_resident() {
+  static int per_cu = 0, cus = 0;
+  if (per_cu == 0) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, dft32_cog_kernel<T8, FAST, FULL, COMP>, 256, 0) !=
+            hipSuccess || per_cu < 1 || cus < 1) {
+      per_cu = 2;
+      cus = 256;
+    }
+  }
+  return (long)per_cu * cus;
 }
 
-hipError_t launch_dft32(const FusedArgs &a, int bf16, hipStream_t st) {
-  if (a.nitems <= 0) return hipSuccess;
-  long blocks = (a.nitems + 3) / 4;
-  // resident workgroups (two waves per SIMD), then grid-stride; three waves per SIMD (168 VGPRs)
-  // measured no faster
-  const long cap = 256L * 2;
+template <typename T8, bool FAST, bool FULL, bool COMP>
+static void launch_dft32_f(const FusedArgs &a, long blocks, hipStream_t st) {
+  // resident workgroups, then grid-stride (the compensated kernels hold two waves per SIMD; three
+  // waves of them, at 168 VGPRs, measured no faster)
+  const long cap = dft32_resident<T8, FAST, FULL, COMP>();
   if (blocks > cap) blocks = cap;
   blocks = (blocks + kXcds - 1) / kXcds * kXcds;   // every XCD gets the same number of workgroups
+  hipLaunchKernelGGL((dft32_cog_kernel<T8, FAST, FULL, COMP>), dim3((unsigned)blocks), dim3(256), 0, st, a);
+}
+
+template <typename T8, bool FAST, bool COMP>
+static void launch_dft32_t(const FusedArgs &a, long blocks, hipStream_t st) {
+  if (a.n == 32) launch_dft32_f<T8, FAST, true, COMP>(a, blocks, st);
+  else launch_dft32_f<T8, FAST, false, COMP>(a, blocks, st);
+}
+
+template <typename T8, bool COMP>
+static void launch_dft32_c(const FusedArgs &a, long blocks, hipStream_t st) {
+  if (a.fast_trig) launch_dft32_t<T8, true, COMP>(a, blocks, st);
+  else launch_dft32_t<T8, false, COMP>(a, blocks, st);
+}
+
+hipError_t launch_dft32(const FusedArgs &a, int bf16, bool comp, hipStream_t st) {
+  if (a.nitems <= 0) return hipSuccess;
+  const long blocks = (a.nitems + 3) / 4;
   if (bf16) {
-    if (a.fast_trig) launch_dft32_t<b8, true>(a, blocks, st);
-    else launch_dft32_t<b8, false>(a, blocks, st);
+    launch_dft32_c<b8, true>(a, blocks, st);   // the single-product form is FP16 only
   } else {
-    if (a.fast_trig) launch_dft32_t<h8, true>(a, blocks, st);
-    else launch_dft32_t<h8, false>(a, blocks, st);
+    if (comp) launch_dft32_c<h8, true>(a, blocks, st);
+    else launch_dft32_c<h8, false>(a, blocks, st);
   }
   return hipGetLastError();
 }

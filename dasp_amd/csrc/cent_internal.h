@@ -302,9 +302,10 @@ hipError_t launch_fill(float *p, long n, float v, hipStream_t st);
 bool fused_supported(int n_pad, int F, int nimg, bool spot);
 hipError_t launch_fused(const FusedArgs &a, int n_pad, int F, bool spot, hipStream_t st);
 bool dft_supported(int n, int F, int nimg);
-hipError_t launch_dft(const FusedArgs &a, int bf16, hipStream_t st);
+// comp: the three-product (compensated) scheme, else one product of the FP16 roundings (bf16 = 0)
+hipError_t launch_dft(const FusedArgs &a, int bf16, bool comp, hipStream_t st);
 // n in 17..32, fftsize 64, nimg 16 (cent_dft32.hip); reached through launch_dft
 bool dft32_supported(int n, int F, int nimg);
-hipError_t launch_dft32(const FusedArgs &a, int bf16, hipStream_t st);
+hipError_t launch_dft32(const FusedArgs &a, int bf16, bool comp, hipStream_t st);
 
 }  // namespace dasp

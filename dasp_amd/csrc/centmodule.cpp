@@ -717,6 +717,7 @@ PyMODINIT_FUNC PyInit_cent(void) {
   PyModule_AddIntConstant(m, "FORM_DFT_BF16", DASP_CENT_FORM_DFT_BF16);
   PyModule_AddIntConstant(m, "FORM_DFT_FP16", DASP_CENT_FORM_DFT_FP16);
   PyModule_AddIntConstant(m, "FORM_FFT32_GENERAL", DASP_CENT_FORM_FFT32_GENERAL);
+  PyModule_AddIntConstant(m, "FORM_DFT_FP16_1", DASP_CENT_FORM_DFT_FP16_1);
   PyModule_AddIntConstant(m, "ABI_VERSION", DASP_CENT_ABI_VERSION);
   PyModule_AddObject(m, "NOALLOC_SCRATCH_BYTES", PyLong_FromLong(DASP_CENT_NOALLOC_SCRATCH_BYTES));
   return m;

@@ -61,6 +61,7 @@ extern "C" {
 #define DASP_CENT_FORM_DFT_BF16 1  /* DFT-matrix GEMMs on MFMA, BF16 hi/lo, FP32 accumulate */
 #define DASP_CENT_FORM_DFT_FP16 2  /* DFT-matrix GEMMs on MFMA, FP16 hi/lo, FP32 accumulate */
 #define DASP_CENT_FORM_FFT32_GENERAL 3 /* FP32 FFT through the general (unfused) kernels only */
+#define DASP_CENT_FORM_DFT_FP16_1 4 /* DFT-matrix GEMMs on MFMA, one product of the FP16 roundings */
 
 typedef struct dasp_cent dasp_cent_t;
 
@@ -156,7 +157,10 @@ int dasp_cent_testcorrelation(const float *img, float *corr, int n);
 
 /* Select the compute form (DASP_CENT_FORM_*). The DFT-GEMM forms cover nimg = 16 with n = 16 /
  * fftsize = clipsize = 32 and n = 17..32 / fftsize = clipsize = 64, the fused kernels' options and
- * no spot kernel; other geometries return DASP_CENT_EUNSUPPORTED. */
+ * no spot kernel; other geometries return DASP_CENT_EUNSUPPORTED. The single-product form
+ * (DASP_CENT_FORM_DFT_FP16_1) rounds each operand once to FP16 (no hi/lo compensation); at n = 16
+ * the detector options outside the inline detector (noise, sky array, relative floor, power,
+ * weights) run the three-product kernel. */
 int dasp_cent_set_form(dasp_cent_t *h, int form);
 
 /* Relative detector floor (no counterpart in the reference, whose readCCD knows threshType 0

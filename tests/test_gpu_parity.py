@@ -305,9 +305,30 @@ def test_run_device_frames_match_legacy():
     c.free()
 
 
-# form codes of tests/_cases.Case: FORM_DFT_FP16 = 2, FORM_DFT_BF16 = 1; + 10 selects the reduced
-# hardware sin/cos (the library's default trig)
-DFT_FORMS = {"fp16": 2, "bf16": 1, "fp16_fasttrig": 12, "bf16_fasttrig": 11}
+# form codes of tests/_cases.Case: FORM_DFT_FP16 = 2, FORM_DFT_BF16 = 1, the single-product FP16
+# form FORM_DFT_FP16_1 = 4; + 10 selects the reduced hardware sin/cos (the library's default trig)
+DFT_FORMS = {"fp16": 2, "bf16": 1, "fp16_fasttrig": 12, "bf16_fasttrig": 11, "fp16x1": 4, "fp16x1_fasttrig": 14}
+
+
+def _strict(form):
+    """The compensated FP16 form is held to the FP32 path's 1e-4 px."""
+    return form.startswith("fp16") and "x1" not in form
+
+
+def _img_tol(form):
+    """Image tolerance (fraction of the image maximum) of each form: the compensated forms' split
+    errors, or one rounding of both operands to FP16 (the single-product form, measured on the
+    80x80 nfft32 phase with an FP64 model of the products: 1.3e-3 of the maximum at the median
+    sub-aperture, 1.6e-3 at the worst; most of it a common scale, as round-toward-zero shrinks
+    every operand)."""
+    if "x1" in form:
+        return 3e-3
+    return 5e-5 if form.startswith("fp16") else 5e-4
+
+
+def _max_bound(form):
+    """Bound on the largest centroid error of the forms not held to 1e-4 px."""
+    return 1e-3 if "x1" in form else 1e-2
 
 
 @pytest.mark.parametrize("form", sorted(DFT_FORMS))
@@ -322,8 +343,7 @@ def test_dft_mfma_form_error_vs_fp64(form):
     d = report("dft-" + form, c.cents, oc, act)
     rms = np.sqrt((d ** 2).mean())
     assert rms < 1e-3
-    if form.startswith("fp16"):
-        assert d.max() < 1e-4
+    assert d.max() < (1e-4 if _strict(form) else _max_bound(form))
     c.free()
 
 
@@ -350,7 +370,7 @@ def test_dft16_options_vs_fp64(name, form):
     act = _active(c)
     say("active sub-apertures: %d" % act.sum())
     b = c.bimg.reshape(-1, 16, 16)[act]
-    tol = 5e-5 if form.startswith("fp16") else 5e-4
+    tol = _img_tol(form)
     # a pixel within the form's error of a threshold may fall on the other side of it
     diffpix = np.abs(b - ob[act]) > tol * np.abs(ob[act]).max()
     say("pixels beyond %.0e of the maximum: %d of %d" % (tol, diffpix.sum(), diffpix.size))
@@ -358,7 +378,7 @@ def test_dft16_options_vs_fp64(name, form):
     clean = ~diffpix.reshape(diffpix.shape[0], -1).any(axis=1)
     d = np.abs(c.cents.reshape(-1, 2)[act] - oc[act])[clean]
     assert np.sqrt((d ** 2).mean()) < 1e-3
-    if form.startswith("fp16") and "noise" not in name:
+    if _strict(form) and "noise" not in name:
         assert d.max() < 1e-4
     assert np.all(c.cents.reshape(-1, 2)[~act] == 0)
     c.free()
@@ -408,9 +428,22 @@ def test_dft32_mfma_form_error_vs_fp64(name, form):
     act = _active(c)
     b = c.bimg.reshape(-1, 16, 16)[act]
     if noise:
-        diffpix = np.abs(b - ob[act]) > 1e-3 * np.maximum(1, np.abs(ob[act]))
+        diff = np.abs(b - ob[act])
+        diffpix = diff > 1e-3 * np.maximum(1, np.abs(ob[act]))
         say("noisy pixels differing: %d of %d" % (diffpix.sum(), diffpix.size))
-        assert diffpix.sum() <= max(3, 1e-3 * diffpix.size)
+        if "x1" in form:
+            # one rounding to 16 bits moves the photon rate by up to _img_tol of the maximum, so
+            # more pixels draw a neighbouring count from the same normal: bound their share, and
+            # each such pixel differs by one or two photons (two where the rate is hundreds of
+            # photons), plus the shift of the relative floor when the brightest pixel moves
+            # (most of them by the floor shift alone: when the brightest pixel of a sub-aperture
+            # moves by a photon, every pixel above the floor moves by a tenth of one)
+            assert diffpix.sum() <= 2e-2 * diffpix.size
+            if diffpix.any():
+                say("largest difference of those pixels: %.3g" % diff[diffpix].max())
+                assert diff[diffpix].max() <= 2.5
+        else:
+            assert diffpix.sum() <= max(3, 1e-3 * diffpix.size)
         clean = ~diffpix.reshape(diffpix.shape[0], -1).any(axis=1)
         d = np.abs(c.cents.reshape(-1, 2)[act] - oc[act])[clean]
         assert np.sqrt((d ** 2).mean()) < 1e-3
@@ -418,10 +451,8 @@ def test_dft32_mfma_form_error_vs_fp64(name, form):
         return
     d = report("dft32-%s-%s" % (name, form), c.cents, oc, act)
     assert np.sqrt((d ** 2).mean()) < 1e-3
-    if form.startswith("fp16"):
-        assert d.max() < 1e-4
-    tol = 5e-5 if form.startswith("fp16") else 5e-4
-    np.testing.assert_allclose(b, ob[act], rtol=0, atol=tol * ob[act].max())
+    assert d.max() < (1e-4 if _strict(form) else _max_bound(form))
+    np.testing.assert_allclose(b, ob[act], rtol=0, atol=_img_tol(form) * ob[act].max())
     assert np.all(c.cents.reshape(-1, 2)[~act] == 0)
     c.free()
 
