@@ -132,6 +132,7 @@ struct dasp_cent {
   bool haveSigArr = false, haveSkyArr = false, haveRef = false, haveCalData = false;
   bool haveCalCoeff = false, haveUBArr = false, haveCorr = false, haveCentWeight = false;
   bool tablesDirty = true;
+  bool allScalePos = false;   // every per-subaperture normalisation scale > 0 (upload_tables)
   bool relFloor = false;   // engine option: threshType 0 floor taken relative to the image maximum
   unsigned long scratchEpoch = 0;   // bumped whenever a scratch buffer of the handle moves
   void *noallocScratch = nullptr;   // the fixed scratch array on this handle's device
@@ -302,6 +303,8 @@ static int upload_tables(dasp_cent *c) {
     skyadd[i] = sky > 0.f ? sky : 0.f;
     pois[i] = (c->addPoisson == 1 && c->calsource == 0) && (c->totsig[i] > 0.f || sky > 0.f || c->haveSkyArr);
   }
+  c->allScalePos = true;
+  for (int i = 0; i < S; ++i) c->allScalePos = c->allScalePos && scale[i] > 0.f;
   if (upload(c->d_scale, scale.data(), S * 4) || upload(c->d_sigv, sigv.data(), S * 4) ||
       upload(c->d_skyadd, skyadd.data(), S * 4) || upload(c->d_poisson, pois.data(), S) ||
       upload(c->d_totsig, c->totsig.data(), S * 4))
@@ -558,6 +561,13 @@ static void fused_args(dasp_cent *c, const float *d_phs, long phs_stride, int nf
   fa.bimg_stride = (long)c->nsubaps * c->nimg * c->nimg;
   fa.slow_detector = (d_zpois && need_pois(c)) || (d_zread && need_read(c)) || c->haveSkyArr ||
                      kthresh(c) == 2 || (c->pxlPower != 1.f && c->pxlPower != 0.f) || c->haveCentWeight;
+  {
+    // the fast detector's pixel is scale * bin + (sky + readbg), then the threshold of type tt
+    const int tt = fa.threshType;
+    const bool adds = c->skybrightness > 0.f || (tt >= 0 && tt <= 2 && c->readbg != 0.f);
+    const bool thr_noop = !(tt == 0 || tt == 1) || (tt == 0 && c->noiseFloor == 0.f) || (tt == 1 && c->noiseFloor <= 0.f);
+    fa.bins_only = !fa.slow_detector && !d_bimg && !adds && thr_noop && c->ncen == c->nimg && c->allScalePos;
+  }
   fa.otfk = c->spotDim ? c->d_otfk.as<float2>() : nullptr;
   fa.otf_per_subap = c->spotDim == 4;
 }

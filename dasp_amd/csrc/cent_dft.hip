@@ -305,10 +305,13 @@ template <typename T8, bool FAST, bool FULLWIN, bool COMP>
 constexpr int kPipeWaves = !COMP ? (FAST ? 3 : 2) : (kPipeThreeWaves<T8, FAST, FULLWIN, COMP> ? 3 : 1);
 
 // COMP: the three-product scheme; without it the single-product form (one product of the FP16
-// roundings, a third of the matrix-core work and no lo parts).
-template <typename T8, bool FAST, bool FULLWIN, bool COMP>
+// roundings, a third of the matrix-core work and no lo parts). BINS (with FULLWIN, when the
+// host's bins_only holds): the CoG is taken on the binned |X|^2 itself, which the positive
+// scale, the absent additions and the no-op threshold leave unchanged up to a factor.
+template <typename T8, bool FAST, bool FULLWIN, bool COMP, bool BINS = false>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kPipeWaves<T8, FAST, FULLWIN, COMP>)))
 void dft16_pipe_kernel(FusedArgs a_) {
+  static_assert(!BINS || FULLWIN, "the bins-only detector takes the whole image as window");
   const FusedArgs &a = a_;
   const int l = threadIdx.x & 63;
   const int h = l >> 5;
@@ -400,7 +403,16 @@ void dft16_pipe_kernel(FusedArgs a_) {
 #pragma unroll
     for (int t = 0; t < 8; ++t) {
       float sn, cs;
-      dft_sincos<FAST>(ph8[t], &sn, &cs);   // the tilt is in B1 and A2
+      if constexpr (FAST && !COMP) {
+        // single-product form: the fraction of a revolution straight from one product (its
+        // rounding, at most 2.4e-5 rad at |phi| = 400 rad, is far inside the FP16 rounding of
+        // the operands, 4.9e-4 relative), then the hardware sin/cos of revolutions
+        const float r = __builtin_amdgcn_fractf(ph8[t] * 0.15915494309189535f);
+        sn = __builtin_amdgcn_sinf(r);
+        cs = __builtin_amdgcn_cosf(r);
+      } else {
+        dft_sincos<FAST>(ph8[t], &sn, &cs);   // the tilt is in B1 and A2
+      }
       er[t] = pu8[t] * cs;
       ei[t] = pu8[t] * sn;
     }
@@ -474,6 +486,16 @@ void dft16_pipe_kernel(FusedArgs a_) {
       const bool valid = 2 * p + sub < cnt;
       const int s = scalar_read(a.active, s0 + 2 * p + (valid ? sub : 0));
       sidx[sub] = s;
+      if constexpr (BINS) {
+        float v[4];
+#pragma unroll
+        for (int qq = 0; qq < 4; ++qq) v[qq] = par ? pix[4 + qq] : pix[qq];
+        const float sw = (v[0] + v[1]) + (v[2] + v[3]);
+        sums[sub][0] = sw;
+        sums[sub][1] = fmaf(fJ0, sw, -fmaf(5.f, v[3], fmaf(4.f, v[2], v[1])));
+        sums[sub][2] = fI * sw;
+        return;
+      }
       const float scale = scalar_read(a.scale, s);
       const float skyadd = scalar_read(a.skyadd, s);
       const float add = (skyadd > 0.f ? skyadd : 0.f) + readbg;
@@ -516,7 +538,7 @@ void dft16_pipe_kernel(FusedArgs a_) {
     load_pair(q + 2 * nwaves, nf, np, ph8, pu8);   // two pairs ahead
     // ---- stores ----
     const int nvalid = 2 * p + 1 < cnt ? 2 : 1;
-    if (a.bimg) {
+    if (!BINS && a.bimg) {   // bins_only holds only without an image buffer
 #pragma unroll
       for (int sub = 0; sub < 2; ++sub) {
         if (sub < nvalid) {
@@ -554,14 +576,14 @@ void dft16_pipe_kernel(FusedArgs a_) {
 // Persistent launch of the pipelined kernel: every wave builds the DFT-matrix operands once, so
 // the grid is the resident wave count (workgroups per CU from the occupancy query, times the CU
 // count) or the pair count if smaller, in whole multiples of the XCD count.
-template <typename T8, bool FAST, bool FULLWIN, bool COMP>
+template <typename T8, bool FAST, bool FULLWIN, bool COMP, bool BINS = false>
 hipError_t launch_pipe_t(const FusedArgs &a, long npairs, hipStream_t st) {
   static int per_cu = 0, cus = 0;
   if (per_cu == 0) {
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess ||
         hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
-        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, dft16_pipe_kernel<T8, FAST, FULLWIN, COMP>, 256, 0) !=
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, dft16_pipe_kernel<T8, FAST, FULLWIN, COMP, BINS>, 256, 0) !=
             hipSuccess || per_cu < 1 || cus < 1) {
       per_cu = 2;
       cus = 256;
@@ -571,12 +593,13 @@ hipError_t launch_pipe_t(const FusedArgs &a, long npairs, hipStream_t st) {
   const long cap = (long)cus * per_cu;
   if (blocks > cap) blocks = cap;
   blocks = (blocks + kXcds - 1) / kXcds * kXcds;   // every XCD gets the same number of workgroups
-  hipLaunchKernelGGL((dft16_pipe_kernel<T8, FAST, FULLWIN, COMP>), dim3((unsigned)blocks), dim3(256), 0, st, a);
+  hipLaunchKernelGGL((dft16_pipe_kernel<T8, FAST, FULLWIN, COMP, BINS>), dim3((unsigned)blocks), dim3(256), 0, st, a);
   return hipGetLastError();
 }
 
 template <typename T8, bool FAST, bool COMP>
 hipError_t launch_pipe_c(const FusedArgs &a, bool full, long npairs, hipStream_t st) {
+  if (full && a.bins_only) return launch_pipe_t<T8, FAST, true, COMP, true>(a, npairs, st);
   return full ? launch_pipe_t<T8, FAST, true, COMP>(a, npairs, st) : launch_pipe_t<T8, FAST, false, COMP>(a, npairs, st);
 }
 

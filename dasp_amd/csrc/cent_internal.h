@@ -202,6 +202,10 @@ struct FusedArgs {
   // the separate inactive kernel does it (multi-WFS step, image-only modes)
   const int *inactive;
   int ninactive;
+  // 1 when the detector is the identity up to a positive scale for every sub-aperture (no
+  // image returned, nothing added, a threshold that changes nothing, the whole image as CoG
+  // window, every scale > 0): the CoG may then be taken on the bins themselves
+  int bins_only;
 };
 
 // Slopes of the inactive sub-apertures set to zero for every frame (centroidsFromPhase gives
