@@ -14,6 +14,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstddef>
+#include <cstdint>
 #include <cstdio>
 #include <cstring>
 #include <algorithm>
@@ -779,6 +780,9 @@ static int check_device_run(dasp_cent *c, const float *d_phs, long phs_stride, i
   if (nframes > 1 && phs_stride < side * side * (c->atmosPhaseType + 1)) return fail(DASP_CENT_EINVAL, "phs_stride too small");
   if (need_pois(c) && !d_zpois) return fail(DASP_CENT_EINVAL, "Poisson noise is on: d_zpois is required");
   if (need_read(c) && !d_zread) return fail(DASP_CENT_EINVAL, "read noise is on: d_zread is required");
+  // the detectors read four consecutive normals per 16-byte load
+  if (((uintptr_t)d_zpois | (uintptr_t)d_zread) % 16 != 0)
+    return fail(DASP_CENT_EINVAL, "d_zpois and d_zread must be 16-byte aligned");
   return 0;
 }
 

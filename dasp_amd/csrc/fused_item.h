@@ -396,20 +396,23 @@ __device__ __forceinline__ void fused_item(const FusedArgs &a, const ItemRef &it
     const bool rn = readnoise != 0.f && zread;
     const bool addbg = threshType >= 0 && threshType <= 2;
     float mx = -INFINITY;
-    // four pixels per trip: their pre-drawn normals are loaded together (one memory latency per
-    // four pixels instead of one per pixel); the pointers are null only when unused
+    // four pixels per trip (consecutive pixels of one detector row): their pre-drawn normals are
+    // two 16-byte loads, issued one trip ahead so that a trip's memory latency overlaps the
+    // previous trip's work; the pointers are null only when unused
     const bool havez = a.zpois != nullptr, haver = zread != nullptr;
     auto pixel = [&](int q) { return det_row(q / NIMG) * NIMG + q % NIMG; };
-    static_assert(PX % 4 == 0, "pixel count per lane is a multiple of 4");
+    static_assert(PX % 4 == 0 && NIMG % 4 == 0, "a lane's pixels come in row-aligned fours");
+    auto loadz = [&](int q0, float4 &zp, float4 &zr) {
+      const int p = pixel(q0);   // pixels p .. p + 3, 16-byte aligned (noff and p are multiples of 4)
+      zp = havez ? *(const float4 *)(zpois + p) : make_float4(0.f, 0.f, 0.f, 0.f);
+      zr = haver ? *(const float4 *)(zread + p) : make_float4(0.f, 0.f, 0.f, 0.f);
+    };
+    float4 zpn, zrn;
+    loadz(0, zpn, zrn);
 #pragma unroll 1
     for (int q0 = 0; q0 < PX; q0 += 4) {
-      float zp4[4], zr4[4];
-#pragma unroll
-      for (int t = 0; t < 4; ++t) {
-        const int p = pixel(q0 + t);
-        zp4[t] = havez ? zpois[p] : 0.f;
-        zr4[t] = haver ? zread[p] : 0.f;
-      }
+      const float zp4[4] = {zpn.x, zpn.y, zpn.z, zpn.w}, zr4[4] = {zrn.x, zrn.y, zrn.z, zrn.w};
+      loadz(q0 + 4 < PX ? q0 + 4 : q0, zpn, zrn);   // the last trip reloads its own pixels
 #pragma unroll
       for (int t = 0; t < 4; ++t) {
         const int q = q0 + t;

@@ -180,7 +180,8 @@ int dasp_cent_set_fast_trig(dasp_cent_t *h, int enable);
  * images and slopes in host memory; here they stay in HBM). Frame f reads
  * d_phs + f*phs_stride and writes d_cents + f*nsubaps*2, d_bimg + f*nsubaps*nimg^2 (if non-NULL)
  * and d_corrimg + f*nsubaps*C^2 (if non-NULL). d_zpois/d_zread are pre-drawn N(0,1) arrays of
- * nsubaps*nimg^2 per frame (may be NULL when the noise they feed is off). Asynchronous on stream.
+ * nsubaps*nimg^2 per frame, 16-byte aligned (may be NULL when the noise they feed is off).
+ * Asynchronous on stream.
  * Every frame is a readout: with nintegrations+nlatency > 1 the caller runs the integration
  * cycle (only the readout step forms an image, from that step's phase, as computeHll zeroes
  * its buffer on every call, cmod/src/centmodule.c:478). */
