@@ -19,7 +19,7 @@ OBJDIR = os.path.join(CSRC, "_build")
 LIB = os.path.join(HERE, "libdasp_cent.so")
 ARCH = "gfx950"
 
-SOURCES = ["cent_form.hip", "cent_backend.hip", "cent_fused.hip", "cent_dft.hip", "cent_dft32.hip", "cent_batch.hip", "cent_batch32.hip", "cent_corr.hip", "cent_interp.hip", "cent_noise.hip", "cent_noalloc.hip", "recon_mvm.hip",
+SOURCES = ["cent_form.hip", "cent_backend.hip", "cent_fused.hip", "cent_dft.hip", "cent_dft32.hip", "cent_batch.hip", "cent_batch32.hip", "cent_corr.hip", "cent_interp.hip", "cent_noise.hip", "cent_noalloc.hip", "recon_mvm.hip", "atmos.hip",
            "cent_api.cpp", "host_math.cpp", "recon_api.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

@@ -17,7 +17,9 @@
 // Extensions (no counterpart in the reference): setForm, setFastTrig, setRelativeFloor, info,
 // runDevice, runDeviceNoAlloc, drawNormals, drawNormalsDevice, hcToSpectrum, setTiming, kernelTime, abiVersion and
 // the multi-WFS step batchCreate / batchTune / batchRun / batchInfo / batchSetConfig / batchFree,
-// and the device reconstructor step reconCreate / reconSetGains / reconRun / reconFree.
+// the device reconstructor step reconCreate / reconSetGains / reconRun / reconFree, and the
+// device phase producer (include/dasp_atmos.h) atmosPupilPhase / atmosLayerUpdate /
+// atmosWorkDoubles / dmSplineOperator / dmSurface / dmLineOfSight.
 #define PY_SSIZE_T_CLEAN
 #include <Python.h>
 #define NPY_NO_DEPRECATED_API NPY_1_7_API_VERSION
@@ -28,6 +30,7 @@
 #include <cstring>
 #include <vector>
 
+#include "../../include/dasp_atmos.h"
 #include "../../include/dasp_cent.h"
 
 static_assert(DASP_CENT_ABI_VERSION == 2, "module written for ABI version 2");
@@ -654,6 +657,126 @@ static PyObject *py_reconFree(PyObject *, PyObject *args) {
   Py_RETURN_NONE;
 }
 
+// ---- device phase producer (atmosphere, deformable mirror) ------------------------------------
+static PyObject *atmos_error(int rc) {
+  const char *m = dasp_atmos_last_error();
+  if (m && m[0]) PyErr_SetString(CentError, m);
+  else PyErr_Format(CentError, "atmosphere/DM error %d", rc);
+  return nullptr;
+}
+
+static PyObject *py_atmosWorkDoubles(PyObject *, PyObject *args) {
+  int npup;
+  if (!PyArg_ParseTuple(args, "i", &npup)) return nullptr;
+  return PyLong_FromLong(dasp_atmos_work_doubles(npup));
+}
+
+// layers: sequence of tuples (screen_ptr, rows, cols, oy, ox, r0, c0, s1, s2, s3, s4)
+static PyObject *py_atmosPupilPhase(PyObject *, PyObject *args, PyObject *kw) {
+  static const char *names[] = {"layers", "npup", "intrinsic_ptr", "pupfn_ptr", "pupil_area", "remove_piston",
+                                "out_scale", "out64_ptr", "out32_ptr", "work_ptr", "stream", nullptr};
+  PyObject *seq;
+  int npup, rp;
+  unsigned long long intr, pup, o64, o32, work, st = 0;
+  double area, scale;
+  if (!PyArg_ParseTupleAndKeywords(args, kw, "OiKKdpdKKK|K", (char **)names, &seq, &npup, &intr, &pup, &area, &rp,
+                                   &scale, &o64, &o32, &work, &st))
+    return nullptr;
+  PyObject *fast = PySequence_Fast(seq, "layers must be a sequence");
+  if (!fast) return nullptr;
+  const Py_ssize_t n = PySequence_Fast_GET_SIZE(fast);
+  if (n > DASP_ATMOS_MAX_LAYERS) {
+    Py_DECREF(fast);
+    PyErr_Format(CentError, "at most %d layers", DASP_ATMOS_MAX_LAYERS);
+    return nullptr;
+  }
+  std::vector<dasp_atmos_layer> L((size_t)n);
+  for (Py_ssize_t k = 0; k < n; ++k) {
+    unsigned long long sp;
+    dasp_atmos_layer &y = L[(size_t)k];
+    if (!PyArg_ParseTuple(PySequence_Fast_GET_ITEM(fast, k), "Kiiiiiiffff", &sp, &y.rows, &y.cols, &y.oy, &y.ox, &y.r0,
+                          &y.c0, &y.s1, &y.s2, &y.s3, &y.s4)) {
+      Py_DECREF(fast);
+      return nullptr;
+    }
+    y.screen = (const double *)(uintptr_t)sp;
+  }
+  Py_DECREF(fast);
+  const int rc = dasp_atmos_pupil_phase(L.data(), (int)n, npup, (const double *)(uintptr_t)intr, (const float *)(uintptr_t)pup,
+                                        area, rp, scale, (double *)(uintptr_t)o64, (float *)(uintptr_t)o32,
+                                        (double *)(uintptr_t)work, (void *)(uintptr_t)st);
+  if (rc != 0) return atmos_error(rc);
+  Py_RETURN_NONE;
+}
+
+static PyObject *py_atmosLayerUpdate(PyObject *, PyObject *args, PyObject *kw) {
+  static const char *names[] = {"screen_ptr", "rows", "cols", "oy", "ox", "cols_ptr", "ncols", "rows_ptr", "nrows",
+                                "col_add", "row_add", "stream", nullptr};
+  unsigned long long scr, cp, rp, st = 0;
+  int rows, cols, oy, ox, nc, nr, ca, ra;
+  if (!PyArg_ParseTupleAndKeywords(args, kw, "KiiiiKiKiii|K", (char **)names, &scr, &rows, &cols, &oy, &ox, &cp, &nc,
+                                   &rp, &nr, &ca, &ra, &st))
+    return nullptr;
+  const int rc = dasp_atmos_layer_update((double *)(uintptr_t)scr, rows, cols, &oy, &ox, (const double *)(uintptr_t)cp, nc,
+                                         (const double *)(uintptr_t)rp, nr, ca, ra, (void *)(uintptr_t)st);
+  if (rc != 0) return atmos_error(rc);
+  return Py_BuildValue("ii", oy, ox);
+}
+
+static PyObject *py_dmSplineOperator(PyObject *, PyObject *args) {
+  int npup, nact;
+  double actoffset = 0.0;
+  if (!PyArg_ParseTuple(args, "ii|d", &npup, &nact, &actoffset)) return nullptr;
+  if (npup < 1 || nact < 2) {
+    PyErr_SetString(CentError, "dmSplineOperator: npup >= 1 and nact >= 2 are required");
+    return nullptr;
+  }
+  npy_intp dims[2] = {npup, nact};
+  PyObject *op = PyArray_SimpleNew(2, dims, NPY_FLOAT64);
+  if (!op) return nullptr;
+  const int rc = dasp_dm_spline_operator(npup, nact, actoffset, (double *)PyArray_DATA((PyArrayObject *)op));
+  if (rc != 0) {
+    Py_DECREF(op);
+    return atmos_error(rc);
+  }
+  return op;
+}
+
+static PyObject *py_dmSurface(PyObject *, PyObject *args, PyObject *kw) {
+  static const char *names[] = {"act_ptr", "nact", "coupling", "op_ptr", "opT_ptr", "npup", "surface_ptr", "work_ptr",
+                                "stream", nullptr};
+  unsigned long long act, op, opT, surf, work, st = 0;
+  int nact, npup;
+  float coupling;
+  if (!PyArg_ParseTupleAndKeywords(args, kw, "KifKKiKK|K", (char **)names, &act, &nact, &coupling, &op, &opT, &npup, &surf,
+                                   &work, &st))
+    return nullptr;
+  const int rc = dasp_dm_surface((const float *)(uintptr_t)act, nact, coupling, (const double *)(uintptr_t)op,
+                                 (const double *)(uintptr_t)opT, npup, (float *)(uintptr_t)surf, (void *)(uintptr_t)work,
+                                 (void *)(uintptr_t)st);
+  if (rc != 0) return atmos_error(rc);
+  Py_RETURN_NONE;
+}
+
+static PyObject *py_dmLineOfSight(PyObject *, PyObject *args, PyObject *kw) {
+  static const char *names[] = {"surface_ptr", "dmpup", "yoff", "xoff", "npup", "factor", "adjust", "base64_ptr",
+                                "base32_ptr", "pupfn_ptr", "pupil_sum", "remove_piston", "out32_ptr", "work_ptr",
+                                "stream", nullptr};
+  unsigned long long surf, b64, b32, pup, out, work, st = 0;
+  int dmpup, yoff, xoff, npup, adjust, rp;
+  float factor;
+  double psum;
+  if (!PyArg_ParseTupleAndKeywords(args, kw, "KiiiifpKKKdpKK|K", (char **)names, &surf, &dmpup, &yoff, &xoff, &npup,
+                                   &factor, &adjust, &b64, &b32, &pup, &psum, &rp, &out, &work, &st))
+    return nullptr;
+  const int rc = dasp_dm_line_of_sight((const float *)(uintptr_t)surf, dmpup, yoff, xoff, npup, factor, adjust,
+                                       (const double *)(uintptr_t)b64, (const float *)(uintptr_t)b32,
+                                       (const float *)(uintptr_t)pup, psum, rp, (float *)(uintptr_t)out,
+                                       (double *)(uintptr_t)work, (void *)(uintptr_t)st);
+  if (rc != 0) return atmos_error(rc);
+  Py_RETURN_NONE;
+}
+
 static PyMethodDef centMethods[] = {
     {"run", py_run, METH_VARARGS, "Run the phase -> centroid algorithm; returns the elapsed seconds."},
     {"runSlope", py_runSlope, METH_VARARGS, "Run the image -> centroid algorithm."},
@@ -691,6 +814,20 @@ static PyMethodDef centMethods[] = {
     {"reconRun", (PyCFunction)(void (*)(void))py_reconRun, METH_VARARGS | METH_KEYWORDS,
      "reconRun(r, slopes_ptr, slope_stride, nframes, acts_ptr, acts_stride, stream=0): acts = decay*acts - gains*(data.reconmx)."},
     {"reconFree", py_reconFree, METH_VARARGS, "reconFree(r)."},
+    {"atmosWorkDoubles", py_atmosWorkDoubles, METH_VARARGS, "atmosWorkDoubles(npup) -> doubles of device work space."},
+    {"atmosPupilPhase", (PyCFunction)(void (*)(void))py_atmosPupilPhase, METH_VARARGS | METH_KEYWORDS,
+     "atmosPupilPhase(layers, npup, intrinsic_ptr, pupfn_ptr, pupil_area, remove_piston, out_scale, out64_ptr, "
+     "out32_ptr, work_ptr, stream=0): createPupilPhs on the device."},
+    {"atmosLayerUpdate", (PyCFunction)(void (*)(void))py_atmosLayerUpdate, METH_VARARGS | METH_KEYWORDS,
+     "atmosLayerUpdate(screen_ptr, rows, cols, oy, ox, cols_ptr, ncols, rows_ptr, nrows, col_add, row_add, stream=0) "
+     "-> new (oy, ox): makeLayer on a ring-buffered screen."},
+    {"dmSplineOperator", py_dmSplineOperator, METH_VARARGS,
+     "dmSplineOperator(npup, nact, actoffset=0) -> (npup, nact) float64 natural-spline operator."},
+    {"dmSurface", (PyCFunction)(void (*)(void))py_dmSurface, METH_VARARGS | METH_KEYWORDS,
+     "dmSurface(act_ptr, nact, coupling, op_ptr, opT_ptr, npup, surface_ptr, work_ptr, stream=0): MirrorSurface.fit."},
+    {"dmLineOfSight", (PyCFunction)(void (*)(void))py_dmLineOfSight, METH_VARARGS | METH_KEYWORDS,
+     "dmLineOfSight(surface_ptr, dmpup, yoff, xoff, npup, factor, adjust, base64_ptr, base32_ptr, pupfn_ptr, "
+     "pupil_sum, remove_piston, out32_ptr, work_ptr, stream=0): selectSubPupil on the device."},
     {nullptr, nullptr, 0, nullptr}};
 
 static struct PyModuleDef centModule = {PyModuleDef_HEAD_INIT, "cent",
