@@ -186,7 +186,7 @@ def main():
     ap.add_argument("--form", default="fft32", choices=sorted(FORM_CODE))
     workloads = {"scao8x8": "SCAO8", "wfs40x40_noise": "NOISE40", "wfs80x80_nfft32": "WFS80",
                  "wfs80x80_nyquist_corr": "CORR80"}   # name -> dasp_amd.workloads attribute
-    ap.add_argument("--workload", default="wfs80x80_nfft32", choices=sorted(workloads) + ["multi_wfs"])
+    ap.add_argument("--workload", default="wfs80x80_nfft32", choices=sorted(workloads) + ["multi_wfs", "closed_loop80"])
     ap.add_argument("--tune-cache", default=None,
                     help="JSON file of autotuned multi-WFS launch configurations (read if present, else written)")
     ap.add_argument("--batch", action="store_true",
@@ -214,6 +214,8 @@ def main():
     torch.cuda.set_device(dev)
     if args.workload == "multi_wfs":
         return bench_multi(args, dev)
+    if args.workload == "closed_loop80":
+        return bench_loop(args, dev)
     w = getattr(W, workloads[args.workload])
     frames = args.frames or w.nframes
     S = w.nsubx * w.nsubx
@@ -449,6 +451,109 @@ def bench_multi(args, dev):
     wb.free()
     for cc in ccs:
         cc.free()
+    print(json.dumps(result), flush=True)
+
+
+def bench_loop(args, dev):
+    """One closed-loop step of the 80x80 system with every array in HBM: three atmosphere layers
+    advanced (makeLayer: one new column and one new row each) and summed into the pupil phase
+    (createPupilPhs, float64), the 81x81-actuator spline mirror refitted and added along the line
+    of sight straight into the WFS's phase buffer, the 80x80 nfft32 WFS (one frame), and the
+    reconstructor step (9264 slopes x 6561 actuators). Phase, slopes and actuators never leave the
+    device; the time per step is the loop's latency."""
+    import torch
+    from dasp_amd import workloads as W
+    from dasp_amd.cmod import cent
+    from dasp_amd.science.atmos import DeviceAtmos
+    from dasp_amd.science.dm import DeviceXinterpDM
+    from dasp_amd.science.tomorecon import DeviceRecon
+    from dasp_amd.util import centcmod
+    w = W.WFS80
+    npup, nact = w.npup, w.nsubx + 1
+    pupil = w.pupil()
+    pfn = pupil.fn.astype(np.float32)
+    host_phs = np.zeros((npup, npup), np.float32)
+    cents_h = np.zeros((w.nsubx, w.nsubx, 2), np.float32)
+    bimg_h = np.zeros((w.nsubx, w.nsubx, w.nimg, w.nimg), np.float32)
+    cc = centcmod.centcmod(1, w.nsubx, w.cen, w.nfft, w.clip, w.nimg, w.n, 0., 0., 0, 0., 1., 0., 0, 1., 1, 0, 1,
+                           host_phs, pupil, None, cents_h, bimg_h, 0.5, 0, None, 0, 0., None, None, 0, 0, 0, 1, 0, None)
+    cent.setFastTrig(cc.centstruct, not args.accurate_trig)
+    rng = np.random.default_rng(1)
+    side = npup + 64
+    layers = [dict(key="L%d" % k, screen=rng.standard_normal((side, side)) * (0.5 / (k + 1)), x0=8 + k, y0=8 + 2 * k,
+                   pos_col=0.0, pos_row=0.0, col_add=-1, row_add=-1 if k % 2 == 0 else 1) for k in range(3)]
+    da = DeviceAtmos(npup, layers, pupfn=pfn, source_lam=640.)
+    dm = DeviceXinterpDM(npup, nact, pfn, actCoupling=0.1)
+    ncents = 2 * int(cc.subflag.sum())
+    R = torch.randn((ncents, nact * nact), device=dev, dtype=torch.float32) * 1e-3
+    rec = DeviceRecon([cc.subflag], R, gains=0.3, decay=0.99)
+    phs32 = torch.zeros((npup, npup), dtype=torch.float32, device=dev)
+    atm64 = torch.empty((npup, npup), dtype=torch.float64, device=dev)
+    new_cols = torch.randn((1, side), dtype=torch.float64, device=dev)
+    new_rows = torch.randn((1, side), dtype=torch.float64, device=dev)
+    stream = torch.cuda.current_stream()
+    st = stream.cuda_stream
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(6)]
+    stage_ms = np.zeros(5)
+
+    def step(k, timed=False):
+        icol = {L["key"]: (0.37 * k) % 1.0 for L in layers}
+        irow = {L["key"]: (0.21 * k) % 1.0 for L in layers}
+        if timed:
+            ev[0].record(stream)
+        for L in layers:
+            da.make_layer(L["key"], new_cols, new_rows)
+        if timed:
+            ev[1].record(stream)
+        da.create_pupil_phs(icol, irow, out64=atm64)
+        if timed:
+            ev[2].record(stream)
+        dm.generateNext(phs32, atmos=atm64, actmap=rec.acts[0].view(nact, nact))
+        if timed:
+            ev[3].record(stream)
+        cent.runDevice(cc.centstruct, phs32.data_ptr(), npup * npup, 1, rec.slope_buffer(0).data_ptr(), stream=st)
+        if timed:
+            ev[4].record(stream)
+        rec.run()
+        if timed:
+            ev[5].record(stream)
+
+    for k in range(args.warmup):
+        step(k)
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    e0.record(stream)
+    for k in range(args.steps):
+        step(args.warmup + k)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms = e0.elapsed_time(e1) / args.steps
+    for k in range(5):   # per-stage times in separately timed steps (events between the stages)
+        step(10_000 + k, timed=True)
+        torch.cuda.synchronize()
+        stage_ms += [ev[i].elapsed_time(ev[i + 1]) for i in range(5)]
+    stage_ms /= 5
+    # createPupilPhs: three (npup + 1)^2 float64 windows read, the sum written and read back, the
+    # pupil read twice, the float64 phase written
+    atm_bytes = 3 * (npup + 1) ** 2 * 8 + 3 * npup * npup * 8 + 2 * npup * npup * 4
+    result = {
+        "metric": "closed-loop step time, 80x80 WFS + 3-layer atmosphere + 81x81 spline DM + reconstructor, all in HBM",
+        "value": ms, "unit": "ms/step", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms,
+        "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "f64 phase, f32 WFS/DM/recon",
+        "data": "synthetic",
+        "config": {"workload": "closed_loop80", "npup": npup, "layers": 3, "nact": nact, "ncents": ncents,
+                   "nacts": nact * nact, "wall_s": wall},
+        "stages_ms": {"make_layer x3": stage_ms[0], "create_pupil_phs": stage_ms[1], "dm_fit_and_line_of_sight": stage_ms[2],
+                      "wfs_80x80": stage_ms[3], "recon": stage_ms[4]},
+        "atmos_roofline": {"bound": "hbm", "achieved": atm_bytes / (stage_ms[1] / 1e3) / 1e9, "peak": HBM_PEAK_GBS,
+                           "unit": "GB/s", "frac": atm_bytes / (stage_ms[1] / 1e3) / 1e9 / HBM_PEAK_GBS,
+                           "algorithmic_bytes": atm_bytes},
+    }
+    rec.free()
+    cc.free()
     print(json.dumps(result), flush=True)
 
 

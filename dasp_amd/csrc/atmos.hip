@@ -117,22 +117,24 @@ __global__ __launch_bounds__(kRowBlock) void atmos_finish_kernel(int npup, const
   if (out32) out32[o] = (float)v;
 }
 
-// makeLayer as a ring update: the new columns (colInput layout: ncols x rows) at logical columns
-// c_first .. c_first + ncols - 1 of every row, then (a second launch, in stream order) the new rows.
-__global__ void ring_cols_kernel(double *scr, int rows, int cols, int oy, int ox, const double *src, int n, int c_first) {
+// makeLayer as a ring update in one launch: the new columns (colInput layout: ncols x rows) at
+// logical columns c_first .. c_first + ncols - 1 of every row and the new rows at logical rows
+// r_first .. r_first + nrows - 1 of every column. The reference writes the columns first and the
+// rows over them, so a column element inside the new rows is left to the row write.
+__global__ void ring_update_kernel(double *scr, int rows, int cols, int oy, int ox, const double *csrc, int nc,
+                                   int c_first, const double *rsrc, int nr, int r_first) {
   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= (long)rows * n) return;
-  const int k = (int)(t / rows), r = (int)(t - (long)k * rows);   // source row k of the new columns
-  const int pr = wrap(r + oy, rows), pc = wrap(c_first + k + ox, cols);
-  scr[(long)pr * cols + pc] = src[(long)k * rows + r];
-}
-
-__global__ void ring_rows_kernel(double *scr, int rows, int cols, int oy, int ox, const double *src, int n, int r_first) {
-  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= (long)cols * n) return;
-  const int k = (int)(t / cols), c = (int)(t - (long)k * cols);
-  const int pr = wrap(r_first + k + oy, rows), pc = wrap(c + ox, cols);
-  scr[(long)pr * cols + pc] = src[(long)k * cols + c];
+  const long ncol_el = (long)rows * nc;
+  if (t < ncol_el) {
+    const int k = (int)(t / rows), r = (int)(t - (long)k * rows);   // source row k of the new columns
+    if (r >= r_first && r < r_first + nr) return;                    // the rows own it
+    scr[(long)wrap(r + oy, rows) * cols + wrap(c_first + k + ox, cols)] = csrc[(long)k * rows + r];
+    return;
+  }
+  const long u = t - ncol_el;
+  if (u >= (long)cols * nr) return;
+  const int k = (int)(u / cols), c = (int)(u - (long)k * cols);
+  scr[(long)wrap(r_first + k + oy, rows) * cols + wrap(c + ox, cols)] = rsrc[(long)k * cols + c];
 }
 
 // MirrorSurface.fudge (util/dm.py:1723-1740) in float32: the neighbour sum in NumPy's order
@@ -163,17 +165,27 @@ __global__ void spline_rows_kernel(const double *op, const float *F, int npup, i
   tmp[t] = v;
 }
 
-// Pass 2: surface[i][k] = sum_b tmp[i][b] opT[b][k], cast to float32. Threads along k read opT
-// rows coalesced; tmp[i][b] is the same for the whole row block.
+// Pass 2: surface[i][k] = sum_b tmp[i][b] opT[b][k], cast to float32. A thread takes four rows i
+// of one column k: four independent accumulations, each opT element loaded once for the four
+// (threads along k read opT rows coalesced; the tmp rows are the same for the whole block).
+constexpr int kSplineRows = 4;
 __global__ __launch_bounds__(kRowBlock) void spline_cols_kernel(const double *tmp, const double *opT, int npup, int nact,
                                                                 float *surface) {
-  const int i = blockIdx.y;
+  const int i0 = blockIdx.y * kSplineRows;
   const int k = blockIdx.x * kRowBlock + threadIdx.x;
   if (k >= npup) return;
-  const double *row = tmp + (long)i * nact;
-  double v = 0.0;
-  for (int b = 0; b < nact; ++b) v = fma(row[b], opT[(long)b * npup + k], v);
-  surface[(long)i * npup + k] = (float)v;
+  const int nr = min(kSplineRows, npup - i0);
+  double v[kSplineRows] = {0.0, 0.0, 0.0, 0.0};
+  const double *row = tmp + (long)i0 * nact;
+  for (int b = 0; b < nact; ++b) {
+    const double o = opT[(long)b * npup + k];
+#pragma unroll
+    for (int r = 0; r < kSplineRows; ++r)
+      if (r < nr) v[r] = fma(row[(long)r * nact + b], o, v[r]);
+  }
+#pragma unroll
+  for (int r = 0; r < kSplineRows; ++r)
+    if (r < nr) surface[(long)(i0 + r) * npup + k] = (float)v[r];
 }
 
 // Line of sight, pass 1: out = base + factor * window (float32 as xinterp_dm's outputData), times
@@ -269,16 +281,10 @@ int dasp_atmos_layer_update(double *d_screen, int rows, int cols, int *oy, int *
   int y = *oy, x = *ox;
   y = row_add < 0 ? (y + nrows) % rows : (y - nrows % rows + rows) % rows;
   x = col_add < 0 ? (x + ncols) % cols : (x - ncols % cols + cols) % cols;
-  if (ncols > 0) {
-    const long n = (long)rows * ncols;
-    hipLaunchKernelGGL(ring_cols_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, d_screen, rows, cols, y,
-                       x, d_cols, ncols, col_add < 0 ? cols - ncols : 0);
-    ACHK(hipGetLastError());
-  }
-  if (nrows > 0) {
-    const long n = (long)cols * nrows;
-    hipLaunchKernelGGL(ring_rows_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, d_screen, rows, cols, y,
-                       x, d_rows, nrows, row_add < 0 ? rows - nrows : 0);
+  const long n = (long)rows * ncols + (long)cols * nrows;
+  if (n > 0) {
+    hipLaunchKernelGGL(ring_update_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, d_screen, rows, cols, y,
+                       x, d_cols, ncols, col_add < 0 ? cols - ncols : 0, d_rows, nrows, row_add < 0 ? rows - nrows : 0);
     ACHK(hipGetLastError());
   }
   *oy = y;
@@ -350,8 +356,8 @@ int dasp_dm_surface(const float *d_act, int nact, float coupling, const double *
   hipLaunchKernelGGL(spline_rows_kernel, dim3((unsigned)((n1 + 255) / 256)), dim3(256), 0, st, d_op, src, npup, nact,
                      tmp);
   ACHK(hipGetLastError());
-  hipLaunchKernelGGL(spline_cols_kernel, dim3(row_blocks(npup), npup), dim3(kRowBlock), 0, st, tmp, d_opT, npup, nact,
-                     d_surface);
+  hipLaunchKernelGGL(spline_cols_kernel, dim3(row_blocks(npup), (npup + kSplineRows - 1) / kSplineRows), dim3(kRowBlock),
+                     0, st, tmp, d_opT, npup, nact, d_surface);
   ACHK(hipGetLastError());
   return 0;
 }

@@ -22,16 +22,41 @@ def _lib():
 
 
 def test_exports_every_declared_symbol():
-    """Both the C library and the extension module carry every entry point of the header."""
+    """Both the C library and the extension module carry every entry point of every header under
+    include/ (dasp_cent.h: engine, multi-WFS step, reconstructor; dasp_atmos.h: phase producer)."""
+    import glob
     cent, lib = _lib()
-    hdr = open(os.path.join(ROOT, "include", "dasp_cent.h")).read()
-    names = sorted(set(re.findall(r"\b(dasp_cent_\w+)\s*\(", hdr)))
-    assert len(names) >= 16
+    names = set()
+    for h in sorted(glob.glob(os.path.join(ROOT, "include", "*.h"))):
+        hdr = re.sub(r"/\*.*?\*/", "", open(h).read(), flags=re.S)   # declarations, not comments
+        names |= set(re.findall(r"\b(dasp_\w+)\s*\(", hdr))
+    names = sorted(names)
+    assert len(names) >= 25 and "dasp_atmos_pupil_phase" in names and "dasp_dm_surface" in names
     ext = ctypes.CDLL(cent.__file__)
     for nm in names:
         assert hasattr(lib, nm), nm
         assert hasattr(ext, nm), nm
     assert lib.dasp_cent_abi_version() == 2 == cent.abiVersion() == cent.ABI_VERSION
+
+
+def test_dm_spline_operator_host_and_argument_checks():
+    """The host-built natural-spline operator equals the oracle's splines of unit vectors; the
+    device entry points refuse bad arguments before touching a GPU."""
+    from dasp_amd.cmod import cent
+    from oracle import atmosoracle as A
+    for npup, nact in ((33, 9), (1280, 81), (98, 85)):
+        op = cent.dmSplineOperator(npup, nact)
+        x, x2 = A.spline_coords(npup, nact)
+        I = np.eye(nact)
+        ref = np.stack([A.natural_spline(x2, I[a], x) for a in range(nact)], axis=1)
+        assert np.abs(op - ref).max() < 1e-12
+    with pytest.raises(cent.error):
+        cent.dmSplineOperator(64, 9, 0.5)        # pixels outside the actuator span (GSL refuses them)
+    with pytest.raises(cent.error):
+        cent.atmosPupilPhase([], 16, 0, 0, 256.0, True, 1.0, 0, 0, 0)   # no output
+    with pytest.raises(cent.error):
+        cent.atmosLayerUpdate(0, 10, 10, 0, 0, 0, 0, 0, 0, -1, -1)      # no screen
+    assert cent.atmosWorkDoubles(1280) == 1280 * 1280 + 1280 * 5 + 1
 
 
 def test_extension_imports_as_cmod_cent(tmp_path):
