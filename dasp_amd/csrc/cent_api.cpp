@@ -149,7 +149,7 @@ struct dasp_cent {
   long phsElems = 0;
   // device state
   DevBuf d_phs, d_pup, d_tilt, d_active, d_winoff, d_inactive, d_scale, d_totsig, d_sigv, d_skyadd, d_skyarr;
-  DevBuf d_poisson, d_binA, d_otf, d_corrK, d_corrPat, d_centWeight, d_refCents, d_calData;
+  DevBuf d_poisson, d_binA, d_otf, d_corrK, d_corrKp, d_corrPat, d_centWeight, d_refCents, d_calData;
   DevBuf d_calBounds, d_calSteps, d_calCoeff, d_ubArr, d_input, d_subapLoc, d_cents, d_bimg;
   DevBuf d_corrimg, d_hll, d_zpois, d_zread, d_bscratch, d_cscratch;
   DevBuf d_otfk;   // spot kernel table of the fused kernels (psfsize == fftsize)
@@ -351,6 +351,15 @@ static int upload_tables(dasp_cent *c) {
     if (!host_fit_matrix(fs, c->fitMx)) return fail(DASP_CENT_EINVAL, "fit matrix for fitsize %d is singular", fs);
     c->fitMxSize = fs;
   }
+  // computeFit's window (fitsize around the peak, clamped to the central ncen x ncen of the
+  // corrsize x corrsize image it fits) has to lie inside that image: the reference reads past the
+  // image otherwise (cmod/src/centmodule.c:1069-1086), here refused
+  if (c->parabolicFit != 0 || c->gaussianFit > 0) {
+    const int win = fs < 0 ? 3 : fs;
+    if (win > c->ncen || c->ncen > c->corrsize)
+      return fail(DASP_CENT_EINVAL, "fit window %d must fit inside ncen %d, and ncen inside the %d x %d image", win,
+                  c->ncen, c->corrsize, c->corrsize);
+  }
   c->tablesDirty = false;
   return 0;
 }
@@ -373,8 +382,30 @@ static int set_corr_pattern(dasp_cent *c, const dasp_cent_array &a) {
       for (size_t q = 0; q < per; ++q) K[per * s + q] = make_float2((float)k[q].real(), (float)k[q].imag());
     }
     if (upload(c->d_corrK, K.data(), K.size() * sizeof(float2))) return fail(DASP_CENT_EHIP, "corr spectrum upload failed");
+    // the two-image correlation kernel's table (C = 8, 16): per row u the columns v = 1..C/2-1 of
+    // the spectrum and, in slots 0 and C/2, (K[u][0] + K[u][C/2]) / 2 and (K[u][0] - K[u][C/2]) / 2:
+    // the packed column carries columns 0 and C/2 of the image as P = X0 + i XH, and
+    // X0 K[u][0] + i XH K[u][C/2] = P (K0 + KH) / 2 + conj(P[-u]) (K0 - KH) / 2
+    c->d_corrKp.release();
+    if (C == 8 || C == 16) {
+      const int H = C / 2;
+      std::vector<float2> Kp((size_t)C * (H + 1) * c->nsubaps);
+      for (int s = 0; s < c->nsubaps; ++s) {
+        host_hc_to_spectrum(src + per * s, C, C, k.data());
+        for (int u = 0; u < C; ++u) {
+          float2 *row = &Kp[((size_t)s * C + u) * (H + 1)];
+          const cd k0 = k[(size_t)u * C], kh = k[(size_t)u * C + H];
+          const cd al = (k0 + kh) * 0.5, be = (k0 - kh) * 0.5;
+          row[0] = make_float2((float)al.real(), (float)al.imag());
+          row[H] = make_float2((float)be.real(), (float)be.imag());
+          for (int v = 1; v < H; ++v) row[v] = K[per * s + (size_t)u * C + v];
+        }
+      }
+      if (upload(c->d_corrKp, Kp.data(), Kp.size() * sizeof(float2))) return fail(DASP_CENT_EHIP, "corr spectrum upload failed");
+    }
   } else {
     c->d_corrK.release();
+    c->d_corrKp.release();
   }
   c->haveCorr = true;
   return 0;
@@ -635,6 +666,7 @@ static int launch_phase(dasp_cent *c, const float *d_phs, long phs_stride, int n
       ca.nimg = c->nimg;
       ca.ncen = c->ncen;
       ca.corrK = corr ? c->d_corrK.as<float2>() : nullptr;
+      ca.corrKp = corr ? c->d_corrKp.as<float2>() : nullptr;
       ca.corrThresh = c->corrThresh;
       ca.doFit = c->parabolicFit != 0 || c->gaussianFit > 0;
       const int fs = c->parabolicFit + c->gaussianFit;
@@ -1270,7 +1302,7 @@ int dasp_cent_free(dasp_cent_t *c) {
   for (hipEvent_t e : c->evs) (void)hipEventDestroy(e);
   c->evs.clear();
   DevBuf *bufs[] = {&c->d_phs, &c->d_pup, &c->d_tilt, &c->d_active, &c->d_winoff, &c->d_inactive, &c->d_scale, &c->d_totsig,
-                    &c->d_sigv, &c->d_skyadd, &c->d_skyarr, &c->d_poisson, &c->d_binA, &c->d_otf, &c->d_corrK,
+                    &c->d_sigv, &c->d_skyadd, &c->d_skyarr, &c->d_poisson, &c->d_binA, &c->d_otf, &c->d_corrK, &c->d_corrKp,
                     &c->d_corrPat, &c->d_centWeight, &c->d_refCents, &c->d_calData, &c->d_calBounds,
                     &c->d_calSteps, &c->d_calCoeff, &c->d_ubArr, &c->d_input, &c->d_subapLoc, &c->d_cents,
                     &c->d_bimg, &c->d_corrimg, &c->d_hll, &c->d_zpois, &c->d_zread, &c->d_bscratch, &c->d_cscratch,

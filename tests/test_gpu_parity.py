@@ -140,6 +140,46 @@ def test_correlation_parabolic_fit():
     c.free()
 
 
+@pytest.mark.parametrize("fit", [0, 3], ids=["centred_cog", "parabolic3"])
+@pytest.mark.parametrize("nsubx", [5, 7])
+def test_correlation_padded_odd_count(nsubx, fit):
+    """FFT correlation with a 16x16 pattern on 8x8 images (centre padding, calcCorrelation
+    cmod/src/centmodule.c:1403-1486) and sub-aperture counts that leave an unpaired image at the
+    end of a slice: the images are correlated two per lane group, the last one alone."""
+    say("case correlation padded nsubx=%d fit=%d" % (nsubx, fit))
+    nimg, C = 8, 16
+    wgeo = W.WFS("corrpad", nsubx, 8, 16, nimg, 4.2, 0.25, 0.2, 30.0)
+    ref = W.reference_images(wgeo)
+    o = (C - nimg) // 2
+    refp = np.zeros((nsubx, nsubx, C, C), np.float32)
+    refp[:, :, o:o + nimg, o:o + nimg] = ref
+    c = Case(nsubx, 8, 16, nimg, obs=0.25, correlationCentroiding=1, corrPattern=refp, parabolicFit=fit,
+             phs=wgeo.phase() * 0.5)
+    oc, ob, ocorr = c.run_both()
+    act = _active(c)
+    d = report("corr padded", c.cents, oc, act)
+    assert d.max() < 1e-4
+    assert np.all(c.cents.reshape(-1, 2)[~act] == 0)
+    ci = c.corrimg.reshape(-1, C, C)[act]
+    np.testing.assert_allclose(ci, ocorr[act], rtol=0, atol=1e-5 * np.abs(ocorr[act]).max())
+    c.free()
+
+
+def test_fit_window_outside_ncen_refused():
+    """computeFit clamps its fitsize window to the central ncen x ncen pixels, which only works when
+    the window fits there (cmod/src/centmodule.c:1069-1086; the reference reads past the image
+    otherwise): a 5x5 fit with ncen 4 is refused when the tables are built, before any kernel."""
+    from dasp_amd.cmod import cent
+    say("case fit window larger than ncen")
+    w = W.SCAO8
+    with pytest.raises((cent.error, RuntimeError)):
+        Case(w.nsubx, w.n, w.nfft, w.nimg, obs=w.obs, ncen=4, parabolicFit=5, phs=w.phase())
+    c = Case(w.nsubx, w.n, w.nfft, w.nimg, obs=w.obs, ncen=5, parabolicFit=5, phs=w.phase())
+    oc, _, _ = c.run_both()
+    assert report("5x5 fit in ncen 5", c.cents, oc, _active(c)).max() < 1e-4
+    c.free()
+
+
 LGS_CASES = {
     # name: (nsubx, n, nfft, nimg, spot size, spot layout, extra)
     "n16_p32_4d": (10, 16, 32, 16, 32, 4, {}),              # fused (psfsize == clipsize == fftsize)
