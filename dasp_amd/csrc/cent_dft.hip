@@ -28,7 +28,9 @@ namespace dasp {
 constexpr int kN = 16, kF = 32, kM = 16;   // phase size, fft size, detector size (bin 2)
 constexpr int kXcds = 8;
 
-template <typename T8, bool FAST>
+// COMP: the three-product scheme; without it the single-product FP16 form (as in the pipelined
+// kernel), which here carries the general detector options (noise, sky arrays, relative floors).
+template <typename T8, bool FAST, bool COMP = true>
 __global__ __launch_bounds__(256) void dft16_cog_kernel(FusedArgs a_) {
   // arguments by value (in scalar registers; some spill to vector-register lanes): reading them
   // through the argument-segment pointer, as the FP32 kernels do, measured 1-2.5 % slower here
@@ -40,7 +42,7 @@ __global__ __launch_bounds__(256) void dft16_cog_kernel(FusedArgs a_) {
   float *px = &pxbuf[threadIdx.x >> 6][l * 8];
   // constant operands: B1 = W^T (B1[i][k] = W[k][i], lane holds k = c, i = 8h + t) and
   // A2 = W with permuted K (A2[l][slot q] = W[l][j], j = 8(q/4) + 4h + q%4, lane holds l = c)
-  CConst<T8> B1, A2;
+  CConst<T8, COMP> B1, A2;
   {
     float re[8], im[8], re2[8], im2[8];
 #pragma unroll
@@ -153,7 +155,7 @@ __global__ __launch_bounds__(256) void dft16_cog_kernel(FusedArgs a_) {
     const int pf0 = cf0, pidx0 = cidx0;
     cf0 = nf0;
     cidx0 = nidx0;
-    COp<T8> A1;
+    COp<T8, COMP> A1;
     A1.set(er, ei);
     f16v zr = {}, zi = {};
     cgemm_vc(A1, B1, zr, zi);   // Z^T stacked: rows (s, j), columns k
@@ -170,7 +172,7 @@ __global__ __launch_bounds__(256) void dft16_cog_kernel(FusedArgs a_) {
         br[q] = zr[8 * sub + q];
         bi[q] = zi[8 * sub + q];
       }
-      COp<T8> B2;
+      COp<T8, COMP> B2;
       B2.set(br, bi);
       f16v xr = {}, xi = {};
       cgemm_cv(A2, B2, xr, xi);   // X^T: rows l, columns k (lane)
@@ -209,26 +211,38 @@ __global__ __launch_bounds__(256) void dft16_cog_kernel(FusedArgs a_) {
         }
         syw = fI * sw;   // the lane's pixels share one detector row
       } else {
-        const bool owner = par == 0;
+        // general detector: each lane of the pair takes four of its eight pixel sums (columns
+        // q = 4p .. 4p+3 as above), so that every lane draws noise for distinct pixels
 #pragma unroll
-        for (int q = 0; q < 8; ++q) px[q] = pix[q];
+        for (int qq = 0; qq < 4; ++qq) px[qq] = par ? pix[4 + qq] : pix[qq];
         const long noff = (long)frame * a.noise_stride + pix0;
         const bool pois = a.poisson_on && a.poisson_on[s];
         const bool rn = a.readnoise != 0.f && a.zread;
+        // the lane's columns are J0, J0-1, J0-4, J0-5 (J0 odd): its pre-drawn normals are two
+        // 8-byte-aligned pairs per array, loaded before the pixels are walked
+        const int J0 = pJ[0];
+        float zp[4] = {0.f, 0.f, 0.f, 0.f}, zr[4] = {0.f, 0.f, 0.f, 0.f};
+        if (pois) {
+          const float2 z0 = *(const float2 *)(a.zpois + noff + I * kM + J0 - 1);
+          const float2 z1 = *(const float2 *)(a.zpois + noff + I * kM + J0 - 5);
+          zp[0] = z0.y; zp[1] = z0.x; zp[2] = z1.y; zp[3] = z1.x;
+        }
+        if (rn) {
+          const float2 z0 = *(const float2 *)(a.zread + noff + I * kM + J0 - 1);
+          const float2 z1 = *(const float2 *)(a.zread + noff + I * kM + J0 - 5);
+          zr[0] = z0.y; zr[1] = z0.x; zr[2] = z1.y; zr[3] = z1.x;
+        }
         float mx = -INFINITY;
-#pragma unroll 1
-        for (int q = 0; q < 8; ++q) {
-          const int J = (kF - 2 - (8 * (q >> 1) + 4 * h + 2 * (q & 1))) >> 1;
-          float v = px[q] * scale;
-          if (owner) {
-            const long p = I * kM + J;
-            if (skyadd > 0.f) v += skyadd;
-            if (a.skyarr) v += a.skyarr[pix0 + p];
-            if (pois) v = poisson_from_normal(v, a.zpois[noff + p]);
-            if (a.threshType >= 0 && a.threshType <= 2) v += rn ? fmaf(a.readnoise, a.zread[noff + p], a.readbg) : a.readbg;
-            mx = fmaxf(mx, v);
-          }
-          px[q] = v;
+#pragma unroll
+        for (int qq = 0; qq < 4; ++qq) {
+          const int J = J0 - (qq == 0 ? 0 : (qq == 1 ? 1 : (qq == 2 ? 4 : 5)));
+          float v = px[qq] * scale;
+          if (skyadd > 0.f) v += skyadd;
+          if (a.skyarr) v += a.skyarr[pix0 + I * kM + J];
+          if (pois) v = poisson_from_normal(v, zp[qq]);
+          if (a.threshType >= 0 && a.threshType <= 2) v += rn ? fmaf(a.readnoise, zr[qq], a.readbg) : a.readbg;
+          mx = fmaxf(mx, v);
+          px[qq] = v;
         }
         float thr = a.floor;
         if (a.threshType == 2) {
@@ -238,21 +252,19 @@ __global__ __launch_bounds__(256) void dft16_cog_kernel(FusedArgs a_) {
         float *bout = a.bimg ? a.bimg + (long)frame * a.bimg_stride + pix0 : nullptr;
         const float *wt = a.centWeight ? a.centWeight + (a.centWeightDim == 4 ? pix0 : 0) : nullptr;
 #pragma unroll 1
-        for (int q = 0; q < 8; ++q) {
-          const int J = (kF - 2 - (8 * (q >> 1) + 4 * h + 2 * (q & 1))) >> 1;
-          float v = px[q];
+        for (int qq = 0; qq < 4; ++qq) {
+          const int J = pJ[0] - (qq == 0 ? 0 : (qq == 1 ? 1 : (qq == 2 ? 4 : 5)));
+          float v = px[qq];
           if (a.threshType == 1) v = v < thr ? 0.f : v;
           else if (a.threshType == 0 || a.threshType == 2) v = v < thr ? 0.f : v - thr;
           if (a.pxlPower == 2.f) v = v * v;
           else if (a.pxlPower != 1.f && a.pxlPower != 0.f) v = powf(v, a.pxlPower);
-          if (owner) {
-            if (bout) bout[I * kM + J] = v;
-            if (I >= st0 && I < en0 && J >= st0 && J < en0) {
-              const float tt = wt ? v * wt[I * kM + J] : v;
-              sw += tt;
-              sxw += tt * J;
-              syw += tt * I;
-            }
+          if (bout) bout[I * kM + J] = v;
+          if (I >= st0 && I < en0 && J >= st0 && J < en0) {
+            const float tt = wt ? v * wt[I * kM + J] : v;
+            sw += tt;
+            sxw += tt * J;
+            syw += tt * I;
           }
         }
       }
@@ -635,7 +647,12 @@ hipError_t launch_dft(const FusedArgs &a, int bf16, bool comp, hipStream_t st) {
     if (a.fast_trig) return launch_pipe<h8, true>(a, full, npairs, st);
     return launch_pipe<h8, false>(a, full, npairs, st);
   }
-  // the general detector options run the three-product kernel in every DFT form
+  // the general detector options: the unpipelined kernel, three products or (FP16) one
+  if (!comp && !bf16) {
+    if (a.fast_trig) hipLaunchKernelGGL((dft16_cog_kernel<h8, true, false>), dim3((unsigned)blocks), dim3(256), 0, st, a);
+    else hipLaunchKernelGGL((dft16_cog_kernel<h8, false, false>), dim3((unsigned)blocks), dim3(256), 0, st, a);
+    return hipGetLastError();
+  }
   if (bf16) {
     if (a.fast_trig) hipLaunchKernelGGL((dft16_cog_kernel<b8, true>), dim3((unsigned)blocks), dim3(256), 0, st, a);
     else hipLaunchKernelGGL((dft16_cog_kernel<b8, false>), dim3((unsigned)blocks), dim3(256), 0, st, a);

@@ -68,22 +68,25 @@ __device__ inline float poisson_small(float lam, float z) {
   // count, the step count is wave-uniform (1/(k+1) comes from the table through the scalar
   // cache) and the wave leaves the loop, checked every four steps, once no lane is searching.
   // The arithmetic of each step is that of the plain search: the same k, bit for bit.
-  float k = 0.f;
+  // The loop is unrolled whole: 1/(k+1) folds into the instructions as literal constants, and the
+  // count is an integer (one add with the flag as carry).
+  int k = 0;
   bool go = u > F;
+#pragma unroll
   for (int i = 0; i < 40; i += 4) {
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
       const float pn = p * (lam * kPoisInv[i + t]);
       const float Fn = F + pn;
       go = go && (Fn > F);
-      k += go ? 1.f : 0.f;
+      k += go ? 1 : 0;
       go = go && (u > Fn);
       p = pn;
       F = Fn;
     }
     if (!__any(go)) break;
   }
-  return k;
+  return (float)k;
 }
 
 // lambda >= 10: the Cornish-Fisher quantile (hardware square root and reciprocal, 1 ulp each: a

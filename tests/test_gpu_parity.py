@@ -412,9 +412,19 @@ def test_dft16_options_vs_fp64(name, form):
     b = c.bimg.reshape(-1, 16, 16)[act]
     tol = _img_tol(form)
     # a pixel within the form's error of a threshold may fall on the other side of it
-    diffpix = np.abs(b - ob[act]) > tol * np.abs(ob[act]).max()
+    diff = np.abs(b - ob[act])
+    diffpix = diff > tol * np.abs(ob[act]).max()
     say("pixels beyond %.0e of the maximum: %d of %d" % (tol, diffpix.sum(), diffpix.size))
-    assert diffpix.sum() <= max(2, 2e-3 * diffpix.size)
+    if "noise" in name and "x1" in form:
+        # one rounding to 16 bits moves the photon rate by up to _img_tol of the maximum, so more
+        # pixels draw a neighbouring count from the same normal (as in the n32 noisy case): bound
+        # their share and their size (a photon or two, plus the relative floor's shift)
+        assert diffpix.sum() <= 2e-2 * diffpix.size
+        if diffpix.any():
+            say("largest difference of those pixels: %.3g" % diff[diffpix].max())
+            assert diff[diffpix].max() <= 2.5
+    else:
+        assert diffpix.sum() <= max(2, 2e-3 * diffpix.size)
     clean = ~diffpix.reshape(diffpix.shape[0], -1).any(axis=1)
     d = np.abs(c.cents.reshape(-1, 2)[act] - oc[act])[clean]
     assert np.sqrt((d ** 2).mean()) < 1e-3
