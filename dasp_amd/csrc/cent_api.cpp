@@ -248,20 +248,27 @@ static int set_spot(dasp_cent *c, const dasp_cent_array &sp) {
   }
   if (upload(c->d_otf, otf.data(), otf.size() * sizeof(float2)) != hipSuccess)
     return fail(DASP_CENT_EHIP, "upload of spot OTFs failed");
-  if (P == c->F && P % 2 == 0) {
-    // Table of the fused spot convolution (spot_convolve in fused_item.h), K = conj(OTF):
-    // rows m = 1..P/2-1: K(u, m); row 0: (K(u,0) + K(u,P/2))/2; row P/2: (K(u,0) - K(u,P/2))/2.
+  if (P == c->F && P % 2 == 0 && c->F == 2 * c->nimg) {
+    // Table of the fused spot convolution (spot_convolve in fused_item.h), K = conj(OTF), for a
+    // detector binned 2 x 2: the block sums over pixel pairs (2m, 2m+1) of an unnormalised
+    // inverse transform are the P/2-point inverse transform of Y[u] f(u) + Y[u + P/2] f(u + P/2),
+    // f(u) = 1 + exp(2 pi i u / P), in both axes, so both fold factors join K:
+    // rows m = 1..P/2-1: K(u, m) f(u) f(m); rows 0 and P/2 (the packed column pair 0, P/2, whose
+    // second member vanishes since f(P/2) = 0): K(u, 0) f(u) f(0) / 2 = K(u, 0) f(u).
     const int Hh = P / 2;
     std::vector<float2> tab((size_t)nk * (Hh + 1) * P);
+    auto fold = [&](int x) { return cd(1.0 + std::cos(2.0 * M_PI * x / P), std::sin(2.0 * M_PI * x / P)); };
     for (int k = 0; k < nk; ++k) {
       const float2 *o = otf.data() + (size_t)k * P * P;
       float2 *t = tab.data() + (size_t)k * (Hh + 1) * P;
       for (int u = 0; u < P; ++u) {
-        const float2 k0 = make_float2(o[u * P].x, -o[u * P].y);
-        const float2 kh = make_float2(o[u * P + Hh].x, -o[u * P + Hh].y);
-        t[u] = make_float2(0.5f * (k0.x + kh.x), 0.5f * (k0.y + kh.y));
-        t[(size_t)Hh * P + u] = make_float2(0.5f * (k0.x - kh.x), 0.5f * (k0.y - kh.y));
-        for (int m = 1; m < Hh; ++m) t[(size_t)m * P + u] = make_float2(o[u * P + m].x, -o[u * P + m].y);
+        const cd k0 = cd(o[u * P].x, -o[u * P].y) * fold(u);
+        t[u] = make_float2((float)k0.real(), (float)k0.imag());
+        t[(size_t)Hh * P + u] = t[u];
+        for (int m = 1; m < Hh; ++m) {
+          const cd km = cd(o[u * P + m].x, -o[u * P + m].y) * fold(u) * fold(m);
+          t[(size_t)m * P + u] = make_float2((float)km.real(), (float)km.imag());
+        }
       }
     }
     if (upload(c->d_otfk, tab.data(), tab.size() * sizeof(float2)) != hipSuccess)

@@ -166,18 +166,25 @@ constexpr int wave_lds_cf() {
 // R a -> a(F-1-.) in both axes, conv(R a, g) = R conv(a, g(-.)) and the spectrum of g(-.) is
 // conj(OTF) for a real g: the unrotated |X|^2 of the lane's rows k0 = 2j, k1 = 2j+1 (z[l].x,
 // z[l].y) is convolved with conj(OTF) and the rotated binning map is kept.
+// The detector bins 2 x 2 (F = 2 nimg), and the block sum over a pixel pair (2m, 2m+1) of an
+// unnormalised inverse transform is the H-point inverse transform (H = F/2) of the folded
+// spectrum Y[u] f(u) + Y[u+H] f(u+H), f(u) = 1 + exp(2 pi i u / F). The host multiplies both fold
+// factors into the kernel table, so the inverse transforms are H-point ones that yield the binned
+// values directly:
 //   rows:    one F-point FFT of the packed pair, Hermitian unpack in registers (v = 0..F/2);
 //   columns: through LDS, lane m owns column v = m (m >= 1) or the packed pair v = 0 (re) and
-//            v = F/2 (im) (m = 0), one F-point FFT, times the kernel, one inverse FFT;
-//   rows:    back through LDS, packed inverse F-point FFT of the two Hermitian row spectra.
-// Kernel table per subaperture (host-built from OTF): row m < NP holds A_m(u) = conj OTF(u, m)
-// for m >= 1 and (K(u,0) + K(u,F/2))/2 for m = 0; row NP holds (K(u,0) - K(u,F/2))/2, with
-// K = conj OTF. Lane m = 0 forms A_0 G(u) + B conj(G(-u)), which is K(u,0) S0(u) + i K(u,F/2)
-// SF(u) for the two Hermitian columns S0, SF packed in G.
+//            v = F/2 (im) (m = 0), one F-point FFT, times the table, fold, one H-point inverse;
+//   rows:    back through LDS, lane j rebuilds binned row j's folded spectrum from its Hermitian
+//            half, one H-point inverse: the row's nimg binned values.
+// Kernel table per subaperture (host-built from OTF, K = conj OTF): row m < NP holds
+// A_m(u) = K(u,m) f(u) f(m) for m >= 1 and K(u,0) f(u) for m = 0, row NP the same as row 0. Lane
+// m = 0 forms A_0 G(u) + A_NP conj(G(-u)) = 2 K(u,0) f(u) S0(u) for the Hermitian column S0 packed
+// in G (the column F/2 drops out: f(F/2) = 0).
 template <int NP, int F, int B, int RB, int NIMG>
 __device__ __forceinline__ void spot_convolve(const FusedArgs &a, int s, int j, cf (&z)[F], cf *T,
                                               float (&acc)[RB][NIMG]) {
   static_assert(F == 2 * NP, "one column per lane needs F == 2 NP");
+  static_assert(B == 2 && RB == 1 && NIMG == F / 2, "the folded inverse transforms bin 2 x 2");
   constexpr int TS2 = NP + 1, H = F / 2;
   const int k0 = 2 * j;
   fft_fwd<F>(z);
@@ -221,29 +228,26 @@ __device__ __forceinline__ void spot_convolve(const FusedArgs &a, int s, int j, 
     o[u] = ou;
     if constexpr (um != u) o[um] = om;
   });
-  fft_inv<F>(o);
-  static_for<0, F>([&](auto kk) { T[decltype(kk)::value * TS2 + j] = o[decltype(kk)::value]; });
+  // fold in u and invert: binned row pairs m' = 0..H-1 of column j
+  cf of[H];
+  static_for<0, H>([&](auto uu) { of[decltype(uu)::value] = cadd(o[decltype(uu)::value], o[decltype(uu)::value + H]); });
+  fft_inv<H>(of);
+  static_for<0, H>([&](auto mm) { T[decltype(mm)::value * TS2 + j] = of[decltype(mm)::value]; });
   wave_lds_sync();
-  cf q[F];
-  {
-    const cf r0 = T[k0 * TS2], r1 = T[(k0 + 1) * TS2];
-    q[0] = {r0.x, r1.x};
-    q[H] = {r0.y, r1.y};
-  }
+  // binned row j: G[v] for v = 0..H-1 from the columns (G[0] real, G[H] = 0 after the fold
+  // factor), the other half by conjugate symmetry; folded: Gf[v] = G[v] + conj(G[H - v])
+  cf q[H];
+  q[0] = {T[j * TS2].x, 0.f};
   static_for<1, H>([&](auto vv) {
     constexpr int v = decltype(vv)::value;
-    const cf r0 = T[k0 * TS2 + v], r1 = T[(k0 + 1) * TS2 + v];
-    q[v] = {r0.x - r1.y, r0.y + r1.x};         // R0 + i R1
-    q[F - v] = {r0.x + r1.y, r1.x - r0.y};     // conj R0 + i conj R1
+    const cf g = T[j * TS2 + v], gc = T[j * TS2 + H - v];
+    q[v] = {g.x + gc.x, g.y - gc.y};
   });
   wave_lds_sync();
-  fft_inv<F>(q);
-  constexpr int R = F / NP;
-  constexpr int ib0 = R >= B ? 0 / B : 0, ib1 = R >= B ? 1 / B : 0;
-  static_for<0, F>([&](auto ll) {
-    constexpr int l = decltype(ll)::value, jb = (F - 1 - l) / B;
-    acc[ib0][jb] += q[l].x;
-    acc[ib1][jb] += q[l].y;
+  fft_inv<H>(q);
+  static_for<0, H>([&](auto nn) {
+    constexpr int n = decltype(nn)::value;
+    acc[0][H - 1 - n] += q[n].x;
   });
 }
 
