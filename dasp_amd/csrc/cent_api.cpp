@@ -18,6 +18,7 @@
 #include <cstdio>
 #include <cstring>
 #include <algorithm>
+#include <functional>
 #include <ctime>
 #include <string>
 #include <vector>
@@ -912,6 +913,10 @@ struct dasp_cent_batch {
   long inactive_total = 0;
   int cus = 0;
   hipStream_t cap = nullptr;
+  // side streams and events of the captured step's fork and join (default priority)
+  static constexpr int kSide = 3;
+  hipStream_t side[kSide] = {nullptr, nullptr, nullptr};
+  hipEvent_t fork = nullptr, join[kSide] = {nullptr, nullptr, nullptr};
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
 };
@@ -1040,28 +1045,67 @@ static int batch_layout(dasp_cent_batch *b) {
 }
 
 // Enqueue one step: the persistent launches, the other WFS's own kernels, the inactive slopes.
-static int batch_enqueue(dasp_cent_batch *b, hipStream_t st) {
-  for (const BatchLaunch &L : b->launches) HIPCHK(batch_launch(L, (int)L.items.size(), L.waves, L.chunk, L.wgs, st));
-  for (int i : b->others) {
-    dasp_cent_batch_item &it = b->items[i];
-    dasp_cent *c = it.handle;
-    Scratch sc;
-    int rc = device_scratch(c, it.nframes, it.bimg, it.corrimg, sc);   // sized at layout time: no allocation here
-    if (rc) return rc;
-    const bool timing = c->timing;
-    c->timing = false;   // no event records inside a captured step
-    rc = launch_phase(c, it.phs, it.phs_stride, it.nframes, it.cents, sc.bimg, sc.corr, it.zpois, it.zread, sc, st);
-    c->timing = timing;
-    if (rc) return rc;
+// The tasks touch disjoint outputs and scratch, so with side streams (the captured step) the
+// largest persistent launch stays on st and the others are forked onto the side streams and
+// joined back: the graph holds them as parallel branches, and the small launches run beside the
+// large one instead of after it.
+static int batch_enqueue(dasp_cent_batch *b, hipStream_t st, bool fork) {
+  std::vector<std::function<int(hipStream_t)>> tasks;
+  size_t big = 0;
+  for (size_t k = 0; k < b->launches.size(); ++k)
+    if (b->launches[k].groups > b->launches[big].groups) big = k;
+  for (size_t k = 0; k < b->launches.size(); ++k) {
+    const BatchLaunch *L = &b->launches[k];
+    auto t = [L](hipStream_t s) -> int {
+      HIPCHK(batch_launch(*L, (int)L->items.size(), L->waves, L->chunk, L->wgs, s));
+      return 0;
+    };
+    if (k == big) tasks.insert(tasks.begin(), t);
+    else tasks.push_back(t);
   }
-  HIPCHK(launch_batch_inactive(b->inactive.as<InactiveDesc>(), b->ninactive_desc, b->inactive_total, st));
-  return 0;
+  for (int i : b->others) {
+    tasks.push_back([b, i](hipStream_t s) -> int {
+      dasp_cent_batch_item &it = b->items[i];
+      dasp_cent *c = it.handle;
+      Scratch sc;
+      int rc = device_scratch(c, it.nframes, it.bimg, it.corrimg, sc);   // sized at layout time: no allocation here
+      if (rc) return rc;
+      const bool timing = c->timing;
+      c->timing = false;   // no event records inside a captured step
+      rc = launch_phase(c, it.phs, it.phs_stride, it.nframes, it.cents, sc.bimg, sc.corr, it.zpois, it.zread, sc, s);
+      c->timing = timing;
+      return rc;
+    });
+  }
+  tasks.push_back([b](hipStream_t s) -> int {
+    HIPCHK(launch_batch_inactive(b->inactive.as<InactiveDesc>(), b->ninactive_desc, b->inactive_total, s));
+    return 0;
+  });
+  if (!fork || tasks.size() == 1) {
+    for (auto &t : tasks) {
+      int rc = t(st);
+      if (rc) return rc;
+    }
+    return 0;
+  }
+  const int nside = (int)std::min<size_t>(dasp_cent_batch::kSide, tasks.size() - 1);
+  HIPCHK(hipEventRecord(b->fork, st));
+  for (int k = 0; k < nside; ++k) HIPCHK(hipStreamWaitEvent(b->side[k], b->fork, 0));
+  int rc = tasks[0](st);
+  for (size_t k = 1; k < tasks.size() && !rc; ++k) rc = tasks[k](b->side[(k - 1) % nside]);
+  // join every side stream back (also after a failure, so that the capture can end cleanly)
+  for (int k = 0; k < nside; ++k) {
+    hipError_t e = hipEventRecord(b->join[k], b->side[k]);
+    if (e == hipSuccess) e = hipStreamWaitEvent(st, b->join[k], 0);
+    if (e != hipSuccess && !rc) rc = fail(DASP_CENT_EHIP, "step fork/join: %s", hipGetErrorString(e));
+  }
+  return rc;
 }
 
 static int batch_capture(dasp_cent_batch *b) {
   batch_drop_graph(b);
   HIPCHK(hipStreamBeginCapture(b->cap, hipStreamCaptureModeThreadLocal));
-  int rc = batch_enqueue(b, b->cap);
+  int rc = batch_enqueue(b, b->cap, true);
   hipGraph_t g = nullptr;
   hipError_t e = hipStreamEndCapture(b->cap, &g);
   if (rc) {
@@ -1687,6 +1731,11 @@ int dasp_cent_batch_create(const dasp_cent_batch_item *items, int nitems, dasp_c
   hipError_t e = hipGetDevice(&dev);
   if (e == hipSuccess) e = hipDeviceGetAttribute(&b->cus, hipDeviceAttributeMultiprocessorCount, dev);
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&b->cap, hipStreamNonBlocking);
+  for (int k = 0; k < dasp_cent_batch::kSide && e == hipSuccess; ++k) {
+    e = hipStreamCreateWithFlags(&b->side[k], hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&b->join[k], hipEventDisableTiming);
+  }
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&b->fork, hipEventDisableTiming);
   if (e != hipSuccess) {
     dasp_cent_batch_free(b);
     return fail(DASP_CENT_EHIP, "batch set-up: %s", hipGetErrorString(e));
@@ -1823,6 +1872,11 @@ int dasp_cent_batch_free(dasp_cent_batch_t *b) {
   }
   b->inactive.release();
   if (b->cap) (void)hipStreamDestroy(b->cap);
+  for (int k = 0; k < dasp_cent_batch::kSide; ++k) {
+    if (b->side[k]) (void)hipStreamDestroy(b->side[k]);
+    if (b->join[k]) (void)hipEventDestroy(b->join[k]);
+  }
+  if (b->fork) (void)hipEventDestroy(b->fork);
   delete b;
   return 0;
 }
