@@ -177,9 +177,9 @@ constexpr int wave_lds_cf() {
 //   rows:    back through LDS, lane j rebuilds binned row j's folded spectrum from its Hermitian
 //            half, one H-point inverse: the row's nimg binned values.
 // Kernel table per subaperture (host-built from OTF, K = conj OTF): row m < NP holds
-// A_m(u) = K(u,m) f(u) f(m) for m >= 1 and K(u,0) f(u) for m = 0, row NP the same as row 0. Lane
-// m = 0 forms A_0 G(u) + A_NP conj(G(-u)) = 2 K(u,0) f(u) S0(u) for the Hermitian column S0 packed
-// in G (the column F/2 drops out: f(F/2) = 0).
+// A_m(u) = K(u,m) f(u) f(m) for m >= 1 and K(u,0) f(u) for m = 0 (row NP, the same as row 0, is
+// not read here). Lane m = 0 forms A_0 (G(u) + conj G(-u)) = 2 K(u,0) f(u) S0(u) for the Hermitian
+// column S0 packed in G (the column F/2 drops out: f(F/2) = 0).
 template <int NP, int F, int B, int RB, int NIMG>
 __device__ __forceinline__ void spot_convolve(const FusedArgs &a, int s, int j, cf (&z)[F], cf *T,
                                               float (&acc)[RB][NIMG]) {
@@ -214,19 +214,17 @@ __device__ __forceinline__ void spot_convolve(const FusedArgs &a, int s, int j, 
   const float2 *kp = a.otfk + (a.otf_per_subap ? (long)s : 0L) * ((NP + 1) * F);
   fft_fwd<F>(c);
   cf o[F];
+  // lane 0's packed column: A_0 G(u) + A_NP conj(G(-u)) = A_0 (G(u) + conj G(-u)) (A_NP = A_0), so
+  // every lane forms (G(u) + m conj G(-u)) times its table row, m = 1 on lane 0 and 0 elsewhere
+  const float m0 = j == 0 ? 1.f : 0.f;
   static_for<0, H + 1>([&](auto uu) {
     constexpr int u = decltype(uu)::value, um = (F - u) % F;
     const cf gu = c[u], gm = c[um];
+    const cf su = {fmaf(m0, gm.x, gu.x), fmaf(-m0, gm.y, gu.y)};
+    const cf sm = {fmaf(m0, gu.x, gm.x), fmaf(-m0, gu.y, gm.y)};
     const float2 au = kp[j * F + u], am = kp[j * F + um];
-    cf ou = cmul(gu, {au.x, au.y});
-    cf om = cmul(gm, {am.x, am.y});
-    if (j == 0) {
-      const float2 bu = kp[NP * F + u], bm = kp[NP * F + um];
-      ou = cadd(ou, cmul({gm.x, -gm.y}, {bu.x, bu.y}));
-      om = cadd(om, cmul({gu.x, -gu.y}, {bm.x, bm.y}));
-    }
-    o[u] = ou;
-    if constexpr (um != u) o[um] = om;
+    o[u] = cmul(su, {au.x, au.y});
+    if constexpr (um != u) o[um] = cmul(sm, {am.x, am.y});
   });
   // fold in u and invert: binned row pairs m' = 0..H-1 of column j
   cf of[H];
