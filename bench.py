@@ -193,6 +193,8 @@ def main():
                     help="run the single WFS through the multi-WFS step (persistent queue kernel + graph)")
     ap.add_argument("--frames", type=int, default=0, help="frames per step (default: the workload's)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-tune", dest="tune", action="store_false",
+                    help="keep the default launch grids (no single-WFS grid autotuning before timing)")
     ap.add_argument("--accurate-trig", action="store_true", help="sincosf instead of the reduced hardware sin/cos")
     ap.add_argument("--no-compare-forms", dest="compare_forms", action="store_false",
                     help="skip timing the other compute forms (geometries the DFT-GEMM forms cover)")
@@ -262,11 +264,20 @@ def main():
         wb.phs[0].copy_(phs)
         wb.tune(reps=3)
 
+    grids = {}
+
     def run_form(form, fast_trig=not args.accurate_trig):
         """Time args.steps steps of one compute form; returns (ms per step, kernel avg s, slopes of frame 0)."""
         cent.setForm(h, FORM_CODE[form])
         cent.setFastTrig(h, fast_trig)
         cents = wb.cents[0].view(frames, S, 2) if wb is not None else d_cents
+        if args.tune and wb is None:
+            # the launch-grid autotuner, before (and outside) the timed steps
+            cent.setGrid(h, 0, 0)
+            gi, gc, _ = cent.tuneGrid(h, phs.data_ptr(), npup * npup, frames, d_cents.data_ptr(),
+                                      zp.data_ptr() if zp is not None else 0, zr.data_ptr() if zr is not None else 0,
+                                      reps=3, stream=stream.cuda_stream)
+            grids[(form, fast_trig)] = [gi, gc]
 
         def step():
             if wb is not None:
@@ -333,6 +344,10 @@ def main():
     }
     if wb is not None:
         result["config"]["path"] = "multi-WFS step (1 WFS): %s" % wb.info()[0][0]
+    if (args.form, not args.accurate_trig) in grids:
+        # image and correlation kernel grids the autotuner chose (0: the default persistent grid,
+        # -1: one workgroup per work unit)
+        result["config"]["launch_grid"] = grids[(args.form, not args.accurate_trig)]
     if kavg:
         result["roofline"] = roofline(args.form, w, nact, frames, kavg, abytes, noise)
     oc = None
@@ -358,6 +373,8 @@ def main():
                 fms, fk, fc = run_form(form)
             e = {"value": nact * frames / (fms / 1e3), "ms_per_step": fms,
                  "roofline": roofline(form, w, nact, frames, fk, abytes, noise) if fk else None}
+            if (form, not args.accurate_trig) in grids:
+                e["launch_grid"] = grids[(form, not args.accurate_trig)]
             if oc is not None and not noise:
                 d = (fc - oc)[act]
                 e["rms_err_vs_fp64_px"] = float(np.sqrt((d ** 2).mean()))

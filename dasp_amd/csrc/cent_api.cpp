@@ -158,6 +158,7 @@ struct dasp_cent {
   std::vector<float> h_stage;
   // kernel timing
   bool timing = false;
+  long gridImage = 0, gridCorr = 0;   // launch grids of the image and correlation kernels (0: default)
   std::vector<hipEvent_t> evs;   // start/stop pairs not yet harvested
   double kernel_ms = 0.0;
   long kernel_launches = 0;
@@ -658,8 +659,8 @@ static int launch_phase(dasp_cent *c, const float *d_phs, long phs_stride, int n
     int trc = timing_mark(c, st);
     if (trc) return trc;
     if (use_dft(c))
-      HIPCHK(launch_dft(fa, c->form == DASP_CENT_FORM_DFT_BF16, c->form != DASP_CENT_FORM_DFT_FP16_1, st));
-    else HIPCHK(launch_fused(fa, np, c->F, c->spotDim != 0, st));
+      HIPCHK(launch_dft(fa, c->form == DASP_CENT_FORM_DFT_BF16, c->form != DASP_CENT_FORM_DFT_FP16_1, st, c->gridImage));
+    else HIPCHK(launch_fused(fa, np, c->F, c->spotDim != 0, st, c->gridImage));
     trc = timing_mark(c, st);   // the timed interval is the image kernel alone
     if (trc) return trc;
     if (stage) {
@@ -690,7 +691,7 @@ static int launch_phase(dasp_cent *c, const float *d_phs, long phs_stride, int n
       ca.cents_stride = (long)c->nsubaps * 2;
       ca.corrimg = corr ? d_corrimg : nullptr;
       ca.corr_stride = (long)c->nsubaps * N * N;
-      HIPCHK(launch_corr_fit(ca, N, corr, st));
+      HIPCHK(launch_corr_fit(ca, N, corr, st, c->gridCorr));
     }
   } else if (nitems > 0) {
     const long hll_elems = nitems * (long)c->padded * c->padded;
@@ -1387,6 +1388,80 @@ int dasp_cent_set_fast_trig(dasp_cent_t *c, int enable) {
   if (!c) return fail(DASP_CENT_EINVAL, "null handle");
   c->gen++;
   c->fastTrig = enable != 0;
+  return 0;
+}
+
+int dasp_cent_set_grid(dasp_cent_t *c, long image_grid, long corr_grid) {
+  if (!c) return fail(DASP_CENT_EINVAL, "null handle");
+  c->gen++;   // a step graph holding the old launches is rebuilt
+  c->gridImage = image_grid;
+  c->gridCorr = corr_grid;
+  return 0;
+}
+
+int dasp_cent_get_grid(dasp_cent_t *c, long *image_grid, long *corr_grid) {
+  if (!c) return fail(DASP_CENT_EINVAL, "null handle");
+  if (image_grid) *image_grid = c->gridImage;
+  if (corr_grid) *corr_grid = c->gridCorr;
+  return 0;
+}
+
+int dasp_cent_tune_grid(dasp_cent_t *c, const float *d_phs, long phs_stride, int nframes, float *d_cents,
+                        const float *d_zpois, const float *d_zread, int reps, void *stream, double *best_ms) {
+  if (!c) return fail(DASP_CENT_EINVAL, "null handle");
+  if (reps < 1) return fail(DASP_CENT_EINVAL, "reps must be positive");
+  hipStream_t st = (hipStream_t)stream;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  HIPCHK(hipEventCreate(&e0));
+  hipError_t he = hipEventCreate(&e1);
+  if (he != hipSuccess) {
+    (void)hipEventDestroy(e0);
+    return fail(DASP_CENT_EHIP, "tune: %s", hipGetErrorString(he));
+  }
+  const bool timing = c->timing;
+  c->timing = false;
+  // one untimed run (scratch allocation, first-call set-up), then reps timed runs
+  auto measure = [&](double &ms) -> int {
+    int rc = dasp_cent_run_device(c, d_phs, phs_stride, nframes, d_cents, nullptr, nullptr, d_zpois, d_zread, st);
+    if (rc) return rc;
+    HIPCHK(hipEventRecord(e0, st));
+    for (int r = 0; r < reps && !rc; ++r)
+      rc = dasp_cent_run_device(c, d_phs, phs_stride, nframes, d_cents, nullptr, nullptr, d_zpois, d_zread, st);
+    if (rc) return rc;
+    HIPCHK(hipEventRecord(e1, st));
+    HIPCHK(hipEventSynchronize(e1));
+    float f = 0.f;
+    HIPCHK(hipEventElapsedTime(&f, e0, e1));
+    ms = f / reps;
+    return 0;
+  };
+  static const long kCand[] = {0, 256, 512, 1024, 2048, 4096, -1};
+  double best = 0.0;
+  int rc = 0;
+  for (int stage = 0; stage < 2 && !rc; ++stage) {
+    if (stage == 1 && !corr_stage(c)) break;
+    long &g = stage == 0 ? c->gridImage : c->gridCorr;
+    long keep = g;
+    double kept = -1.0;
+    for (long cand : kCand) {
+      g = cand;
+      double ms = 0.0;
+      rc = measure(ms);
+      if (rc) break;
+      if (kept < 0.0 || ms < kept) {
+        kept = ms;
+        keep = cand;
+      }
+    }
+    g = keep;
+    best = kept;
+  }
+  c->timing = timing;
+  c->gen++;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (rc) return rc;
+  if (best_ms) *best_ms = best;
   return 0;
 }
 

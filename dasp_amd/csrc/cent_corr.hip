@@ -616,18 +616,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(N <= 8 ? 4 
 
 bool corr_fit_supported(int n) { return n == 8 || n == 16 || n == 32; }
 
-hipError_t launch_corr_fit(const CorrArgs &a, int n, bool corr, hipStream_t st) {
+hipError_t launch_corr_fit(const CorrArgs &a, int n, bool corr, hipStream_t st, long grid) {
   const long ipb = 4L * (64 / n);
-  long blocks = (a.nitems + ipb - 1) / ipb;
-  if (blocks <= 0) return hipSuccess;
-  if (blocks > 2048) blocks = 2048;
-  blocks = (blocks + kXcds - 1) / kXcds * kXcds;   // every XCD gets the same number of workgroups
+  if (a.nitems <= 0) return hipSuccess;
+  const long blocks = launch_grid((a.nitems + ipb - 1) / ipb, 2048, grid);
   // correlation (mode 1): two items per lane group through shared transforms (corr_pair_kernel)
   if (corr && a.corrKp && (n == 8 || n == 16)) {
     const long ppb = 4L * (64 / n), pairs = (a.nitems + 1) / 2 + kXcds;
-    long pb = (pairs + ppb - 1) / ppb;
-    if (pb > 2048) pb = 2048;
-    pb = (pb + kXcds - 1) / kXcds * kXcds;
+    const long pb = launch_grid((pairs + ppb - 1) / ppb, 2048, grid);
     const bool pad = a.nimg != n;
     if (n == 8) {
       if (pad) hipLaunchKernelGGL((corr_pair_kernel<8, true>), dim3((unsigned)pb), dim3(256), 0, st, a);

@@ -589,7 +589,7 @@ void dft16_pipe_kernel(FusedArgs a_) {
 // the grid is the resident wave count (workgroups per CU from the occupancy query, times the CU
 // count) or the pair count if smaller, in whole multiples of the XCD count.
 template <typename T8, bool FAST, bool FULLWIN, bool COMP, bool BINS = false>
-hipError_t launch_pipe_t(const FusedArgs &a, long npairs, hipStream_t st) {
+hipError_t launch_pipe_t(const FusedArgs &a, long npairs, hipStream_t st, long grid) {
   static int per_cu = 0, cus = 0;
   if (per_cu == 0) {
     int dev = 0;
@@ -601,51 +601,46 @@ hipError_t launch_pipe_t(const FusedArgs &a, long npairs, hipStream_t st) {
       cus = 256;
     }
   }
-  long blocks = (npairs + 3) / 4;
-  const long cap = (long)cus * per_cu;
-  if (blocks > cap) blocks = cap;
-  blocks = (blocks + kXcds - 1) / kXcds * kXcds;   // every XCD gets the same number of workgroups
+  const long blocks = launch_grid((npairs + 3) / 4, (long)cus * per_cu, grid);
   hipLaunchKernelGGL((dft16_pipe_kernel<T8, FAST, FULLWIN, COMP, BINS>), dim3((unsigned)blocks), dim3(256), 0, st, a);
   return hipGetLastError();
 }
 
 template <typename T8, bool FAST, bool COMP>
-hipError_t launch_pipe_c(const FusedArgs &a, bool full, long npairs, hipStream_t st) {
-  if (full && a.bins_only) return launch_pipe_t<T8, FAST, true, COMP, true>(a, npairs, st);
-  return full ? launch_pipe_t<T8, FAST, true, COMP>(a, npairs, st) : launch_pipe_t<T8, FAST, false, COMP>(a, npairs, st);
+hipError_t launch_pipe_c(const FusedArgs &a, bool full, long npairs, hipStream_t st, long grid) {
+  if (full && a.bins_only) return launch_pipe_t<T8, FAST, true, COMP, true>(a, npairs, st, grid);
+  return full ? launch_pipe_t<T8, FAST, true, COMP>(a, npairs, st, grid)
+              : launch_pipe_t<T8, FAST, false, COMP>(a, npairs, st, grid);
 }
 
 template <typename T8, bool FAST>
-hipError_t launch_pipe(const FusedArgs &a, bool full, long npairs, hipStream_t st) {
-  return launch_pipe_c<T8, FAST, true>(a, full, npairs, st);
+hipError_t launch_pipe(const FusedArgs &a, bool full, long npairs, hipStream_t st, long grid) {
+  return launch_pipe_c<T8, FAST, true>(a, full, npairs, st, grid);
 }
 
 bool dft_supported(int n, int F, int nimg) {
   return (n == kN && F == kF && nimg == kM) || dft32_supported(n, F, nimg);
 }
 
-hipError_t launch_dft(const FusedArgs &a, int bf16, bool comp, hipStream_t st) {
+hipError_t launch_dft(const FusedArgs &a, int bf16, bool comp, hipStream_t st, long grid) {
   if (a.nitems <= 0) return hipSuccess;
-  if (a.n != kN) return launch_dft32(a, bf16, comp, st);
+  if (a.n != kN) return launch_dft32(a, bf16, comp, st, grid);
   const long npairs = (a.nitems + 1) / 2 + kXcds;
-  long blocks = (npairs + 3) / 4;
   // persistent grid for the general-detector kernel: every wave builds the DFT-matrix operands
   // once (hundreds of instructions), so launch its resident wave count (two waves per SIMD)
-  const long cap = 256L * 2;
-  if (blocks > cap) blocks = cap;
-  blocks = (blocks + kXcds - 1) / kXcds * kXcds;   // every XCD gets the same number of workgroups
+  const long blocks = launch_grid((npairs + 3) / 4, 256L * 2, grid);
   if (!a.slow_detector) {
     const bool full = a.ncen == kM;
     if (bf16) {
-      if (a.fast_trig) return launch_pipe<b8, true>(a, full, npairs, st);
-      return launch_pipe<b8, false>(a, full, npairs, st);
+      if (a.fast_trig) return launch_pipe<b8, true>(a, full, npairs, st, grid);
+      return launch_pipe<b8, false>(a, full, npairs, st, grid);
     }
     if (!comp) {   // the single-product form is FP16 only
-      if (a.fast_trig) return launch_pipe_c<h8, true, false>(a, full, npairs, st);
-      return launch_pipe_c<h8, false, false>(a, full, npairs, st);
+      if (a.fast_trig) return launch_pipe_c<h8, true, false>(a, full, npairs, st, grid);
+      return launch_pipe_c<h8, false, false>(a, full, npairs, st, grid);
     }
-    if (a.fast_trig) return launch_pipe<h8, true>(a, full, npairs, st);
-    return launch_pipe<h8, false>(a, full, npairs, st);
+    if (a.fast_trig) return launch_pipe<h8, true>(a, full, npairs, st, grid);
+    return launch_pipe<h8, false>(a, full, npairs, st, grid);
   }
   // the general detector options: the unpipelined kernel, three products or (FP16) one
   if (!comp && !bf16) {

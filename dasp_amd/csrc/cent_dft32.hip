@@ -320,35 +320,33 @@ static long dft32_resident() {
 }
 
 template <typename T8, bool FAST, bool FULL, bool COMP>
-static void launch_dft32_f(const FusedArgs &a, long blocks, hipStream_t st) {
+static void launch_dft32_f(const FusedArgs &a, long blocks, hipStream_t st, long grid) {
   // resident workgroups, then grid-stride (the compensated kernels hold two waves per SIMD; three
-  // waves of them, at 168 VGPRs, measured no faster)
-  const long cap = dft32_resident<T8, FAST, FULL, COMP>();
-  if (blocks > cap) blocks = cap;
-  blocks = (blocks + kXcds - 1) / kXcds * kXcds;   // every XCD gets the same number of workgroups
+  // waves of them, at 168 VGPRs, measured no faster), or the tuned grid
+  blocks = launch_grid(blocks, dft32_resident<T8, FAST, FULL, COMP>(), grid);
   hipLaunchKernelGGL((dft32_cog_kernel<T8, FAST, FULL, COMP>), dim3((unsigned)blocks), dim3(256), 0, st, a);
 }
 
 template <typename T8, bool FAST, bool COMP>
-static void launch_dft32_t(const FusedArgs &a, long blocks, hipStream_t st) {
-  if (a.n == 32) launch_dft32_f<T8, FAST, true, COMP>(a, blocks, st);
-  else launch_dft32_f<T8, FAST, false, COMP>(a, blocks, st);
+static void launch_dft32_t(const FusedArgs &a, long blocks, hipStream_t st, long grid) {
+  if (a.n == 32) launch_dft32_f<T8, FAST, true, COMP>(a, blocks, st, grid);
+  else launch_dft32_f<T8, FAST, false, COMP>(a, blocks, st, grid);
 }
 
 template <typename T8, bool COMP>
-static void launch_dft32_c(const FusedArgs &a, long blocks, hipStream_t st) {
-  if (a.fast_trig) launch_dft32_t<T8, true, COMP>(a, blocks, st);
-  else launch_dft32_t<T8, false, COMP>(a, blocks, st);
+static void launch_dft32_c(const FusedArgs &a, long blocks, hipStream_t st, long grid) {
+  if (a.fast_trig) launch_dft32_t<T8, true, COMP>(a, blocks, st, grid);
+  else launch_dft32_t<T8, false, COMP>(a, blocks, st, grid);
 }
 
-hipError_t launch_dft32(const FusedArgs &a, int bf16, bool comp, hipStream_t st) {
+hipError_t launch_dft32(const FusedArgs &a, int bf16, bool comp, hipStream_t st, long grid) {
   if (a.nitems <= 0) return hipSuccess;
   const long blocks = (a.nitems + 3) / 4;
   if (bf16) {
-    launch_dft32_c<b8, true>(a, blocks, st);   // the single-product form is FP16 only
+    launch_dft32_c<b8, true>(a, blocks, st, grid);   // the single-product form is FP16 only
   } else {
-    if (comp) launch_dft32_c<h8, true>(a, blocks, st);
-    else launch_dft32_c<h8, false>(a, blocks, st);
+    if (comp) launch_dft32_c<h8, true>(a, blocks, st, grid);
+    else launch_dft32_c<h8, false>(a, blocks, st, grid);
   }
   return hipGetLastError();
 }

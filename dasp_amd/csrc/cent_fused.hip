@@ -76,16 +76,13 @@ void fused_cog_kernel(FusedArgs a_) {
 }
 
 template <int NP, int F, int B, bool SPOT>
-static hipError_t launch_fused_t(const FusedArgs &a, hipStream_t st) {
+static hipError_t launch_fused_t(const FusedArgs &a, hipStream_t st, long grid) {
   using S = FusedShape<NP, F, B>;
   if (a.nitems <= 0 || a.nactive <= 0) return hipSuccess;
   const long nsg = (a.nactive + S::SPW - 1) / S::SPW;
   const long groups = nsg * (a.nitems / a.nactive);
-  long blocks = (groups + S::WAVES - 1) / S::WAVES;
-  // persistent grid: enough workgroups for every CU to hold several, a multiple of the XCD count
-  const long cap = 256L * 4;
-  if (blocks > cap) blocks = cap;
-  blocks = (blocks + kXcds - 1) / kXcds * kXcds;
+  // persistent grid: enough workgroups for every CU to hold several (or the tuned grid)
+  const long blocks = launch_grid((groups + S::WAVES - 1) / S::WAVES, 256L * 4, grid);
   if (a.fast_trig)
     hipLaunchKernelGGL((fused_cog_kernel<NP, F, B, true, SPOT>), dim3((unsigned)blocks), dim3(64 * S::WAVES), 0, st, a);
   else
@@ -111,15 +108,15 @@ bool fused_supported(int np, int F, int nimg, bool spot) {
   return false;
 }
 
-hipError_t launch_fused(const FusedArgs &a, int np, int F, bool spot, hipStream_t st) {
+hipError_t launch_fused(const FusedArgs &a, int np, int F, bool spot, hipStream_t st, long grid) {
   // the argument block comes from another translation unit: refuse, rather than divide by, a
   // detector size that cannot be right
   if (a.nimg <= 0 || F % a.nimg != 0) return hipErrorInvalidValue;
   const int b = F / a.nimg;
 #define DASP_FUSED_CASE(A, Bf, C) \
-  if (np == A && F == Bf && b == C) return launch_fused_t<A, Bf, C, false>(a, st);
+  if (np == A && F == Bf && b == C) return launch_fused_t<A, Bf, C, false>(a, st, grid);
 #define DASP_FUSED_SPOT_CASE(A, Bf, C) \
-  if (np == A && F == Bf && b == C) return launch_fused_t<A, Bf, C, true>(a, st);
+  if (np == A && F == Bf && b == C) return launch_fused_t<A, Bf, C, true>(a, st, grid);
   if (spot) {
     DASP_FUSED_SPOT_SHAPES(DASP_FUSED_SPOT_CASE)
   } else {

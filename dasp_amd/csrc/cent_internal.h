@@ -262,7 +262,19 @@ hipError_t launch_interp(const InterpArgs &a, hipStream_t st);
 hipError_t launch_normals(uint64_t seed, uint64_t stream, long n, float *out, hipStream_t st);
 
 bool corr_fit_supported(int n);
-hipError_t launch_corr_fit(const CorrArgs &a, int n, bool corr, hipStream_t st);
+hipError_t launch_corr_fit(const CorrArgs &a, int n, bool corr, hipStream_t st, long grid = 0);
+
+// Workgroups of a persistent grid-stride launch: `natural` (one per work unit) capped at `cap`
+// (the kernel's default persistent grid); a tuned override g > 0 caps at g instead, g < 0 lifts
+// the cap. Rounded up to a whole multiple of the 8 XCDs (workgroup b runs on XCD b % 8, and the
+// kernels split their work by XCD, so every XCD gets the same number of workgroups).
+inline long launch_grid(long natural, long cap, long g) {
+  constexpr long xcds = 8;
+  const long c = g > 0 ? g : (g < 0 ? natural : cap);
+  long b = natural < c ? natural : c;
+  if (b < 1) b = 1;
+  return (b + xcds - 1) / xcds * xcds;
+}
 
 // Multi-WFS persistent step (batch_kernel.h): one descriptor per WFS in a shape's launch.
 struct BatchDesc {
@@ -305,12 +317,12 @@ hipError_t launch_backend(const BackArgs &a, hipStream_t st);
 hipError_t launch_inactive(const InactiveArgs &a, hipStream_t st);
 hipError_t launch_fill(float *p, long n, float v, hipStream_t st);
 bool fused_supported(int n_pad, int F, int nimg, bool spot);
-hipError_t launch_fused(const FusedArgs &a, int n_pad, int F, bool spot, hipStream_t st);
+hipError_t launch_fused(const FusedArgs &a, int n_pad, int F, bool spot, hipStream_t st, long grid = 0);
 bool dft_supported(int n, int F, int nimg);
 // comp: the three-product (compensated) scheme, else one product of the FP16 roundings (bf16 = 0)
-hipError_t launch_dft(const FusedArgs &a, int bf16, bool comp, hipStream_t st);
+hipError_t launch_dft(const FusedArgs &a, int bf16, bool comp, hipStream_t st, long grid = 0);
 // n in 17..32, fftsize 64, nimg 16 (cent_dft32.hip); reached through launch_dft
 bool dft32_supported(int n, int F, int nimg);
-hipError_t launch_dft32(const FusedArgs &a, int bf16, bool comp, hipStream_t st);
+hipError_t launch_dft32(const FusedArgs &a, int bf16, bool comp, hipStream_t st, long grid = 0);
 
 }  // namespace dasp

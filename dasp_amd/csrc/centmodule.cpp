@@ -14,7 +14,8 @@
 // also holds a reference to each of them until free(), so they cannot be collected under it.
 // Failures raise cent.error with the engine's message (the reference printed and returned NULL).
 //
-// Extensions (no counterpart in the reference): setForm, setFastTrig, setRelativeFloor, info,
+// Extensions (no counterpart in the reference): setForm, setFastTrig, setRelativeFloor, setGrid,
+// getGrid, tuneGrid, info,
 // runDevice, runDeviceNoAlloc, drawNormals, drawNormalsDevice, hcToSpectrum, setTiming, kernelTime, abiVersion and
 // the multi-WFS step batchCreate / batchTune / batchRun / batchInfo / batchSetConfig / batchFree,
 // the device reconstructor step reconCreate / reconSetGains / reconRun / reconFree, and the
@@ -358,6 +359,51 @@ static PyObject *py_runDevice(PyObject *, PyObject *args, PyObject *kw) {
   Py_END_ALLOW_THREADS
   if (rc != 0) return engine_error(rc);
   Py_RETURN_NONE;
+}
+
+static PyObject *py_setGrid(PyObject *, PyObject *args, PyObject *kw) {
+  static const char *names[] = {"handle", "image", "corr", nullptr};
+  PyObject *ho;
+  long gi = 0, gc = 0;
+  dasp_cent_t *h;
+  if (!PyArg_ParseTupleAndKeywords(args, kw, "O|ll", (char **)names, &ho, &gi, &gc) || !to_handle(ho, &h)) return nullptr;
+  const int rc = dasp_cent_set_grid(h, gi, gc);
+  if (rc != 0) return engine_error(rc);
+  Py_RETURN_NONE;
+}
+
+static PyObject *py_getGrid(PyObject *, PyObject *args) {
+  PyObject *ho;
+  dasp_cent_t *h;
+  if (!PyArg_ParseTuple(args, "O", &ho) || !to_handle(ho, &h)) return nullptr;
+  long gi = 0, gc = 0;
+  const int rc = dasp_cent_get_grid(h, &gi, &gc);
+  if (rc != 0) return engine_error(rc);
+  return Py_BuildValue("(ll)", gi, gc);
+}
+
+static PyObject *py_tuneGrid(PyObject *, PyObject *args, PyObject *kw) {
+  static const char *names[] = {"handle", "phs_ptr", "phs_stride", "nframes", "cents_ptr", "zpois_ptr", "zread_ptr",
+                                "reps", "stream", nullptr};
+  PyObject *ho;
+  unsigned long long phs, cents, zp = 0, zr = 0, st = 0;
+  long stride;
+  int nframes, reps = 5;
+  dasp_cent_t *h;
+  if (!PyArg_ParseTupleAndKeywords(args, kw, "OKliK|KKiK", (char **)names, &ho, &phs, &stride, &nframes, &cents, &zp,
+                                   &zr, &reps, &st) ||
+      !to_handle(ho, &h))
+    return nullptr;
+  int rc;
+  double ms = 0.0;
+  Py_BEGIN_ALLOW_THREADS
+    rc = dasp_cent_tune_grid(h, (const float *)as_ptr(phs), stride, nframes, (float *)as_ptr(cents),
+                             (const float *)as_ptr(zp), (const float *)as_ptr(zr), reps, as_ptr(st), &ms);
+  Py_END_ALLOW_THREADS
+  if (rc != 0) return engine_error(rc);
+  long gi = 0, gc = 0;
+  (void)dasp_cent_get_grid(h, &gi, &gc);
+  return Py_BuildValue("(lld)", gi, gc, ms);
 }
 
 static PyObject *py_runDeviceNoAlloc(PyObject *, PyObject *args, PyObject *kw) {
@@ -789,6 +835,12 @@ static PyMethodDef centMethods[] = {
     {"setRelativeFloor", (PyCFunction)(void (*)(void))py_setRelativeFloor, METH_VARARGS | METH_KEYWORDS,
      "setRelativeFloor(handle, enable=True): threshType 0 floor = noiseFloor * max(image) when 0 < noiseFloor <= 1."},
     {"info", py_info, METH_VARARGS, "info(handle) -> (nactive, npup, corrsize)."},
+    {"setGrid", (PyCFunction)(void (*)(void))py_setGrid, METH_VARARGS | METH_KEYWORDS,
+     "setGrid(handle, image=0, corr=0): launch grids of the image and correlation kernels (0: default, <0: uncapped)."},
+    {"getGrid", (PyCFunction)(void (*)(void))py_getGrid, METH_VARARGS, "getGrid(handle) -> (image_grid, corr_grid)."},
+    {"tuneGrid", (PyCFunction)(void (*)(void))py_tuneGrid, METH_VARARGS | METH_KEYWORDS,
+     "tuneGrid(handle, phs_ptr, phs_stride, nframes, cents_ptr, zpois_ptr=0, zread_ptr=0, reps=5, stream=0) -> "
+     "(image_grid, corr_grid, ms): time the launch grids on these inputs and keep the fastest."},
     {"runDevice", (PyCFunction)(void (*)(void))py_runDevice, METH_VARARGS | METH_KEYWORDS,
      "runDevice(handle, phs_ptr, phs_stride, nframes, cents_ptr, bimg_ptr=0, corrimg_ptr=0, zpois_ptr=0, "
      "zread_ptr=0, stream=0): "

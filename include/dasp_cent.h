@@ -158,9 +158,9 @@ int dasp_cent_testcorrelation(const float *img, float *corr, int n);
 /* Select the compute form (DASP_CENT_FORM_*). The DFT-GEMM forms cover nimg = 16 with n = 16 /
  * fftsize = clipsize = 32 and n = 17..32 / fftsize = clipsize = 64, the fused kernels' options and
  * no spot kernel; other geometries return DASP_CENT_EUNSUPPORTED. The single-product form
- * (DASP_CENT_FORM_DFT_FP16_1) rounds each operand once to FP16 (no hi/lo compensation); at n = 16
+ * (DASP_CENT_FORM_DFT_FP16_1) rounds each operand once to FP16 (no hi/lo compensation), also with
  * the detector options outside the inline detector (noise, sky array, relative floor, power,
- * weights) run the three-product kernel. */
+ * weights). */
 int dasp_cent_set_form(dasp_cent_t *h, int form);
 
 /* Relative detector floor (no counterpart in the reference, whose readCCD knows threshType 0
@@ -175,6 +175,23 @@ int dasp_cent_set_relative_floor(dasp_cent_t *h, int enable);
  * most 2.6e-7 rad for |x| <= 400 rad), then the hardware sine/cosine (v_sin_f32/v_cos_f32);
  * 0 = sincosf (max abs error 7e-8). The general kernels use sincosf. */
 int dasp_cent_set_fast_trig(dasp_cent_t *h, int enable);
+
+/* Launch grids of the single-WFS kernels: image_grid for the image kernel (fused or DFT-GEMM
+ * form), corr_grid for the correlation/fit stage; 0 = the kernel's default persistent grid,
+ * g > 0 = at most g workgroups, g < 0 = one workgroup per work unit (no cap), rounded up to a
+ * multiple of 8 (one share per XCD). Every item is computed the same way whatever the grid: the
+ * results do not depend on it. */
+int dasp_cent_set_grid(dasp_cent_t *h, long image_grid, long corr_grid);
+int dasp_cent_get_grid(dasp_cent_t *h, long *image_grid, long *corr_grid);
+
+/* Autotuner of the single-WFS launches: times dasp_cent_run_device on the given device inputs
+ * (one untimed run, then reps timed runs, HIP events on stream) for the image grids {default,
+ * 256, 512, 1024, 2048, 4096 workgroups, uncapped}, keeps the fastest, then does the same for the
+ * correlation/fit stage's grid when that stage runs; the choice stays in the handle
+ * (dasp_cent_get_grid). Writes the slopes as dasp_cent_run_device does and synchronises stream.
+ * *best_ms (may be NULL): the tuned step time in milliseconds. */
+int dasp_cent_tune_grid(dasp_cent_t *h, const float *d_phs, long phs_stride, int nframes, float *d_cents,
+                        const float *d_zpois, const float *d_zread, int reps, void *stream, double *best_ms);
 
 /* Device-resident step over nframes phase frames (science/wfscent.py:630-702 keeps phase,
  * images and slopes in host memory; here they stay in HBM). Frame f reads

@@ -545,6 +545,58 @@ def test_device_noise_stream_matches_host(seed, stream, n):
     np.testing.assert_allclose(got, host, rtol=4 * np.finfo(np.float32).eps, atol=1e-30)
 
 
+@pytest.mark.parametrize("form", ["fused", "fused_noise", "dft16_fp16x1", "dft32_fp16x1_corr"])
+def test_launch_grid_tuner_invariance(form):
+    """dasp_cent_set_grid / dasp_cent_tune_grid: every item is computed the same way whatever the
+    launch grid, so the slopes of a multi-frame device run are bit for bit those of the default
+    grid under capped, uncapped and tuned grids, for the image kernel and the correlation stage."""
+    import torch
+    from dasp_amd.cmod import cent
+    say("case launch grid tuner %s" % form)
+    kw = {}
+    if form == "dft32_fp16x1_corr":
+        wgeo = W.WFS("dft32", 12, 32, 64, 16, 8.0, 0.2, 0.15, 25.0)
+        kw = dict(correlationCentroiding=1, corrPattern=W.reference_images(wgeo), parabolicFit=5)
+        c = Case(12, 32, 64, 16, obs=0.2, phs=wgeo.phase(), form=4, **kw)
+    else:
+        wgeo = W.WFS("dft16", 11, 16, 32, 16, 8.0, 0.2, 0.15, 25.0)
+        if form == "fused_noise":
+            kw = dict(sig=1000.0, readnoise=0.5, addPoisson=1, noiseFloor=0.1, threshType=0, relativeFloor=True)
+        c = Case(11, 16, 32, 16, obs=0.2, phs=wgeo.phase(), form=4 if form == "dft16_fp16x1" else 10, **kw)
+    h = c.cc.centstruct
+    nframes = 5
+    frames = np.stack([wgeo.phase(f) for f in range(nframes)])
+    dev = torch.device("cuda:0")
+    ph = torch.from_numpy(frames).to(dev)
+    S = c.nsubx * c.nsubx
+    nact, npup, csz = cent.info(h)
+    npix = S * c.nimg * c.nimg * nframes
+    zp = torch.from_numpy(np.random.default_rng(7).standard_normal(npix).astype(np.float32)).to(dev)
+    zr = torch.from_numpy(np.random.default_rng(8).standard_normal(npix).astype(np.float32)).to(dev)
+    st = torch.cuda.current_stream().cuda_stream
+
+    def run():
+        ce = torch.full((nframes, S, 2), -9.0, dtype=torch.float32, device=dev)
+        cent.runDevice(h, ph.data_ptr(), npup * npup, nframes, ce.data_ptr(), 0, 0, zp.data_ptr(), zr.data_ptr(), st)
+        torch.cuda.synchronize()
+        return ce.cpu().numpy()
+
+    cent.setGrid(h, 0, 0)
+    want = run()
+    for gi, gc in [(8, 8), (256, 16), (-1, -1), (4096, 2048)]:
+        cent.setGrid(h, gi, gc)
+        assert cent.getGrid(h) == (gi, gc)
+        np.testing.assert_array_equal(run(), want)
+    ce = torch.zeros((nframes, S, 2), dtype=torch.float32, device=dev)
+    gi, gc, ms = cent.tuneGrid(h, ph.data_ptr(), npup * npup, nframes, ce.data_ptr(), zp.data_ptr(), zr.data_ptr(),
+                               reps=2, stream=st)
+    say("tuned grids %d / %d, %.4f ms" % (gi, gc, ms))
+    assert ms > 0 and cent.getGrid(h) == (gi, gc)
+    np.testing.assert_array_equal(ce.cpu().numpy(), want)
+    np.testing.assert_array_equal(run(), want)
+    c.free()
+
+
 @pytest.mark.parametrize("form", ["fused", "dft16_fp16", "dft16_bf16_fasttrig", "dft32_fp16_corr"])
 def test_run_device_zeroes_inactive_slopes(form):
     """The image kernels zero the inactive sub-apertures' slopes of every frame themselves (no
