@@ -160,6 +160,10 @@ struct dasp_cent {
   bool timing = false;
   long gridImage = 0, gridCorr = 0;   // launch grids of the image and correlation kernels (0: default)
   std::vector<hipEvent_t> evs;   // start/stop pairs not yet harvested
+  // side stream of the correlation stage, which runs frame chunks beside the image kernel's next
+  // chunk (created on first use, default priority), and its fork / join events
+  hipStream_t corrSide = nullptr;
+  hipEvent_t corrFork = nullptr, corrJoin = nullptr;
   double kernel_ms = 0.0;
   long kernel_launches = 0;
 };
@@ -391,26 +395,29 @@ static int set_corr_pattern(dasp_cent *c, const dasp_cent_array &a) {
       for (size_t q = 0; q < per; ++q) K[per * s + q] = make_float2((float)k[q].real(), (float)k[q].imag());
     }
     if (upload(c->d_corrK, K.data(), K.size() * sizeof(float2))) return fail(DASP_CENT_EHIP, "corr spectrum upload failed");
-    // the two-image correlation kernel's table (C = 8, 16): per row u the columns v = 1..C/2-1 of
-    // the spectrum and, in slots 0 and C/2, (K[u][0] + K[u][C/2]) / 2 and (K[u][0] - K[u][C/2]) / 2:
-    // the packed column carries columns 0 and C/2 of the image as P = X0 + i XH, and
-    // X0 K[u][0] + i XH K[u][C/2] = P (K0 + KH) / 2 + conj(P[-u]) (K0 - KH) / 2
+    // the two-image correlation kernel's table (C = 8, 16): per row u and column v < C/2 the pair
+    // (alpha, beta) of W = alpha P[u] + beta conj(P[-u]): alpha = K[u][v], beta = 0 for v >= 1; for
+    // v = 0 the packed column carries columns 0 and C/2 of the image as P = X0 + i XH, and
+    // X0 K[u][0] + i XH K[u][C/2] = P (K0 + KH) / 2 + conj(P[-u]) (K0 - KH) / 2. One 16-byte load
+    // per value for every lane.
     c->d_corrKp.release();
     if (C == 8 || C == 16) {
       const int H = C / 2;
-      std::vector<float2> Kp((size_t)C * (H + 1) * c->nsubaps);
+      std::vector<float4> Kp((size_t)C * H * c->nsubaps);
       for (int s = 0; s < c->nsubaps; ++s) {
         host_hc_to_spectrum(src + per * s, C, C, k.data());
         for (int u = 0; u < C; ++u) {
-          float2 *row = &Kp[((size_t)s * C + u) * (H + 1)];
+          float4 *row = &Kp[((size_t)s * C + u) * H];
           const cd k0 = k[(size_t)u * C], kh = k[(size_t)u * C + H];
           const cd al = (k0 + kh) * 0.5, be = (k0 - kh) * 0.5;
-          row[0] = make_float2((float)al.real(), (float)al.imag());
-          row[H] = make_float2((float)be.real(), (float)be.imag());
-          for (int v = 1; v < H; ++v) row[v] = K[per * s + (size_t)u * C + v];
+          row[0] = make_float4((float)al.real(), (float)al.imag(), (float)be.real(), (float)be.imag());
+          for (int v = 1; v < H; ++v) {
+            const float2 kv = K[per * s + (size_t)u * C + v];
+            row[v] = make_float4(kv.x, kv.y, 0.f, 0.f);
+          }
         }
       }
-      if (upload(c->d_corrKp, Kp.data(), Kp.size() * sizeof(float2))) return fail(DASP_CENT_EHIP, "corr spectrum upload failed");
+      if (upload(c->d_corrKp, Kp.data(), Kp.size() * sizeof(float4))) return fail(DASP_CENT_EHIP, "corr spectrum upload failed");
     }
   } else {
     c->d_corrK.release();
@@ -656,17 +663,10 @@ static int launch_phase(dasp_cent *c, const float *d_phs, long phs_stride, int n
       fa.ninactive = (int)c->inactive.size();
       inactive_done = true;
     }
-    int trc = timing_mark(c, st);
-    if (trc) return trc;
-    if (use_dft(c))
-      HIPCHK(launch_dft(fa, c->form == DASP_CENT_FORM_DFT_BF16, c->form != DASP_CENT_FORM_DFT_FP16_1, st, c->gridImage));
-    else HIPCHK(launch_fused(fa, np, c->F, c->spotDim != 0, st, c->gridImage));
-    trc = timing_mark(c, st);   // the timed interval is the image kernel alone
-    if (trc) return trc;
+    CorrArgs ca{};
+    const bool corr = c->correlationCentroiding == 1;
+    const int N = corr ? c->corrsize : c->nimg;
     if (stage) {
-      CorrArgs ca{};
-      const bool corr = c->correlationCentroiding == 1;
-      const int N = corr ? c->corrsize : c->nimg;
       ca.img = d_bimg;
       ca.img_stride = (long)c->nsubaps * c->nimg * c->nimg;
       ca.active = c->d_active.as<int>();
@@ -675,7 +675,7 @@ static int launch_phase(dasp_cent *c, const float *d_phs, long phs_stride, int n
       ca.nimg = c->nimg;
       ca.ncen = c->ncen;
       ca.corrK = corr ? c->d_corrK.as<float2>() : nullptr;
-      ca.corrKp = corr ? c->d_corrKp.as<float2>() : nullptr;
+      ca.corrKp = corr ? c->d_corrKp.as<float4>() : nullptr;
       ca.corrThresh = c->corrThresh;
       ca.doFit = c->parabolicFit != 0 || c->gaussianFit > 0;
       const int fs = c->parabolicFit + c->gaussianFit;
@@ -691,6 +691,57 @@ static int launch_phase(dasp_cent *c, const float *d_phs, long phs_stride, int n
       ca.cents_stride = (long)c->nsubaps * 2;
       ca.corrimg = corr ? d_corrimg : nullptr;
       ca.corr_stride = (long)c->nsubaps * N * N;
+    }
+    // With the correlation/fit stage and several frames, the frames go in chunks: the image
+    // kernel of chunk k+1 (matrix-core and memory heavy) runs on st while the stage of chunk k
+    // (vector heavy) runs on a side stream; the side stream joins st at the end. Not in the
+    // allocation-free entry point (no stream or event creation there), and not while st is being
+    // captured unless the side stream already exists.
+    int nch = 1;
+    if (stage && nframes >= 2 && !t_forbid_alloc) {
+      hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+      const bool capturing = hipStreamIsCapturing(st, &cs) == hipSuccess && cs != hipStreamCaptureStatusNone;
+      if (!c->corrSide && !capturing) {
+        if (hipStreamCreateWithFlags(&c->corrSide, hipStreamNonBlocking) != hipSuccess) c->corrSide = nullptr;
+        if (c->corrSide && (hipEventCreateWithFlags(&c->corrFork, hipEventDisableTiming) != hipSuccess ||
+                            hipEventCreateWithFlags(&c->corrJoin, hipEventDisableTiming) != hipSuccess)) {
+          return fail(DASP_CENT_EHIP, "correlation side stream set-up failed");
+        }
+      }
+      if (c->corrSide && c->corrFork && c->corrJoin) nch = nframes >= 16 ? 4 : (nframes >= 4 ? 2 : 1);
+    }
+    const long nact = (long)c->active.size();
+    int trc = timing_mark(c, st);
+    if (trc) return trc;
+    for (int k = 0; k < nch; ++k) {
+      const int f0 = (int)((long)nframes * k / nch), f1 = (int)((long)nframes * (k + 1) / nch);
+      FusedArgs fk = fa;
+      fk.phs = fa.phs + (long)f0 * fa.phs_stride;
+      fk.nitems = (long)(f1 - f0) * nact;
+      if (fk.cents) fk.cents = fa.cents + (long)f0 * fa.cents_stride;
+      if (fk.bimg) fk.bimg = fa.bimg + (long)f0 * fa.bimg_stride;
+      if (fk.zpois) fk.zpois = fa.zpois + (long)f0 * fa.noise_stride;
+      if (fk.zread) fk.zread = fa.zread + (long)f0 * fa.noise_stride;
+      if (use_dft(c))
+        HIPCHK(launch_dft(fk, c->form == DASP_CENT_FORM_DFT_BF16, c->form != DASP_CENT_FORM_DFT_FP16_1, st, c->gridImage));
+      else HIPCHK(launch_fused(fk, np, c->F, c->spotDim != 0, st, c->gridImage));
+      if (nch > 1) {
+        CorrArgs ck = ca;
+        ck.img = ca.img + (long)f0 * ca.img_stride;
+        ck.nitems = (long)(f1 - f0) * nact;
+        ck.cents = ca.cents + (long)f0 * ca.cents_stride;
+        if (ck.corrimg) ck.corrimg = ca.corrimg + (long)f0 * ca.corr_stride;
+        HIPCHK(hipEventRecord(c->corrFork, st));
+        HIPCHK(hipStreamWaitEvent(c->corrSide, c->corrFork, 0));
+        HIPCHK(launch_corr_fit(ck, N, corr, c->corrSide, c->gridCorr));
+      }
+    }
+    trc = timing_mark(c, st);   // the timed interval is the image kernel alone
+    if (trc) return trc;
+    if (nch > 1) {
+      HIPCHK(hipEventRecord(c->corrJoin, c->corrSide));
+      HIPCHK(hipStreamWaitEvent(st, c->corrJoin, 0));
+    } else if (stage) {
       HIPCHK(launch_corr_fit(ca, N, corr, st, c->gridCorr));
     }
   } else if (nitems > 0) {
@@ -1353,6 +1404,9 @@ int dasp_cent_free(dasp_cent_t *c) {
   if (!c) return 0;
   for (hipEvent_t e : c->evs) (void)hipEventDestroy(e);
   c->evs.clear();
+  if (c->corrSide) (void)hipStreamDestroy(c->corrSide);
+  if (c->corrFork) (void)hipEventDestroy(c->corrFork);
+  if (c->corrJoin) (void)hipEventDestroy(c->corrJoin);
   DevBuf *bufs[] = {&c->d_phs, &c->d_pup, &c->d_tilt, &c->d_active, &c->d_winoff, &c->d_inactive, &c->d_scale, &c->d_totsig,
                     &c->d_sigv, &c->d_skyadd, &c->d_skyarr, &c->d_poisson, &c->d_binA, &c->d_otf, &c->d_corrK, &c->d_corrKp,
                     &c->d_corrPat, &c->d_centWeight, &c->d_refCents, &c->d_calData, &c->d_calBounds,

@@ -467,19 +467,17 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(N <= 8 ? 4 
     static_for<0, N>([&](auto u) { y[decltype(u)::value] = T[decltype(u)::value * TS + j]; });
     wave_sync();
     fft_fwd<N>(y);
-    // W[u] = alpha[u] Y[u] + beta[u] conj(Y[-u]): alpha = K[u][v], beta = 0 on a regular column;
-    // on the packed column (v = 0) the table's (K0 + KH) / 2 and (K0 - KH) / 2, which multiply
-    // columns 0 and H by their own spectra and pack them again (one branch-free form for all lanes)
-    const float2 *kp = a.corrKp + (long)(isB ? cur.sB : cur.sA) * N * (H + 1) + v;
-    const float bm = v == 0 ? 1.f : 0.f;
+    // W[u] = alpha[u] Y[u] + beta[u] conj(Y[-u]) from the table's (alpha, beta): alpha = K[u][v],
+    // beta = 0 on a regular column; on the packed column (v = 0) (K0 + KH) / 2 and (K0 - KH) / 2,
+    // which multiply columns 0 and H by their own spectra and pack them again (one branch-free
+    // form and one 16-byte load per value for all lanes)
+    const float4 *kp = a.corrKp + (long)(isB ? cur.sB : cur.sA) * N * H + v;
     cf wv[N];
     static_for<0, N>([&](auto uu) {
       constexpr int u = decltype(uu)::value, um = (N - u) % N;
-      const float2 al = kp[u * (H + 1)];
-      const float2 b0 = kp[u * (H + 1) + H - v];
-      const float bx = b0.x * bm, by = b0.y * bm;
+      const float4 ab = kp[u * H];
       const cf p = y[u], pm = y[um];
-      wv[u] = {p.x * al.x - p.y * al.y + pm.x * bx + pm.y * by, p.x * al.y + p.y * al.x + pm.x * by - pm.y * bx};
+      wv[u] = {p.x * ab.x - p.y * ab.y + pm.x * ab.z + pm.y * ab.w, p.x * ab.y + p.y * ab.x + pm.x * ab.w - pm.y * ab.z};
     });
     static_for<0, N>([&](auto u) { y[decltype(u)::value] = wv[decltype(u)::value]; });
     fft_inv<N>(y);

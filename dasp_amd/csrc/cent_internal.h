@@ -232,7 +232,7 @@ struct CorrArgs {
   long nitems;
   int nimg, ncen;
   const float2 *corrK;   // (nsubaps, N, N) pattern spectrum, mode 1
-  const float2 *corrKp;  // (nsubaps, N, N/2 + 1) two-image table for N = 8, 16 (or null)
+  const float4 *corrKp;  // (nsubaps, N, N/2) two-image table (alpha, beta) for N = 8, 16 (or null)
   float corrThresh;
   int doFit, fitsize, gaussianFit;
   float gaussianMinVal, gaussianReplaceVal;
