@@ -397,6 +397,10 @@ __device__ __forceinline__ void fused_item(const FusedArgs &a, const ItemRef &it
     const int threshType = a.threshType;
     const bool rn = readnoise != 0.f && zread;
     const bool addbg = threshType >= 0 && threshType <= 2;
+    const float sky = skyadd > 0.f ? skyadd : 0.f;
+    const float bg = addbg ? readbg : 0.f;
+    const bool owner_pois = owner && pois;
+    const bool rn_bg = rn && addbg;   // read noise only where readCCD adds the background
     float mx = -INFINITY;
     // four pixels per trip (consecutive pixels of one detector row): their pre-drawn normals are
     // two 16-byte loads, issued one trip ahead so that a trip's memory latency overlaps the
@@ -437,6 +441,25 @@ __device__ __forceinline__ void fused_item(const FusedArgs &a, const ItemRef &it
     }
     float *bout = a.bimg ? a.bimg + (long)frame * a.bimg_stride + pix0 : nullptr;
     const float *wt = a.centWeight ? a.centWeight + (a.centWeightDim == 4 ? pix0 : 0) : nullptr;
+    if (pxlPower == 1.f && !wt) {
+      // no pixel power and no weights (the usual noisy detector): the threshold by selects and
+      // the window by a per-lane row test and compile-time columns, unrolled
+      const bool t1 = threshType == 1, t02 = threshType == 0 || threshType == 2;
+      const float tsub = t02 ? thr : 0.f;
+      const bool thr_on = t1 || t02;
+      static_for<0, PX>([&](auto qq) {
+        constexpr int q = decltype(qq)::value, jb = q % NIMG;
+        const int I = det_row(q / NIMG);
+        float v = px[q];
+        v = (thr_on && v < thr) ? 0.f : v - tsub;
+        if (bout && owner) bout[I * NIMG + jb] = v;
+        const bool in = owner && I >= st0 && I < en0 && jb >= st0 && jb < en0;
+        const float t = in ? v : 0.f;
+        sw += t;
+        sxw = fmaf(t, (float)jb, sxw);
+        syw = fmaf(t, (float)I, syw);
+      });
+    } else {
 #pragma unroll 1
     for (int q = 0; q < PX; ++q) {
       const int I = det_row(q / NIMG), jb = q % NIMG;
@@ -454,6 +477,7 @@ __device__ __forceinline__ void fused_item(const FusedArgs &a, const ItemRef &it
           syw += t * I;
         }
       }
+    }
     }
     wave_lds_sync();   // the region holds the next item's transposes
   }
