@@ -251,6 +251,23 @@ __global__ __launch_bounds__(256) void dft16_cog_kernel(FusedArgs a_) {
         }
         float *bout = a.bimg ? a.bimg + (long)frame * a.bimg_stride + pix0 : nullptr;
         const float *wt = a.centWeight ? a.centWeight + (a.centWeightDim == 4 ? pix0 : 0) : nullptr;
+        if (a.pxlPower == 1.f && !wt) {
+          // no pixel power and no weights (the usual noisy detector): selects, unrolled
+          const bool t1 = a.threshType == 1, t02 = a.threshType == 0 || a.threshType == 2;
+          const bool thr_on = t1 || t02;
+          const float tsub = t02 ? thr : 0.f;
+#pragma unroll
+          for (int qq = 0; qq < 4; ++qq) {
+            const int J = J0 - (qq == 0 ? 0 : (qq == 1 ? 1 : (qq == 2 ? 4 : 5)));
+            float v = px[qq];
+            v = (thr_on && v < thr) ? 0.f : v - tsub;
+            if (bout) bout[I * kM + J] = v;
+            const float tt = (I >= st0 && I < en0 && J >= st0 && J < en0) ? v : 0.f;
+            sw += tt;
+            sxw = fmaf(tt, (float)J, sxw);
+            syw = fmaf(tt, (float)I, syw);
+          }
+        } else
 #pragma unroll 1
         for (int qq = 0; qq < 4; ++qq) {
           const int J = pJ[0] - (qq == 0 ? 0 : (qq == 1 ? 1 : (qq == 2 ? 4 : 5)));
