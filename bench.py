@@ -23,8 +23,9 @@ HBM_PEAK_GBS = 8000.0
 MFMA_PEAK_TFLOPS = {"bf16": 2500.0, "fp16": 2500.0, "fp16x1": 2500.0}
 # engine form codes (DASP_CENT_FORM_*): the compensated DFT forms (three products of hi/lo
 # splits) and the single-product FP16 one (x1)
-FORM_CODE = {"fft32": 0, "bf16": 1, "fp16": 2, "fp16x1": 4}
-FORM_DTYPE = {"fft32": "f32", "bf16": "bf16x3+f32acc", "fp16": "f16x3+f32acc", "fp16x1": "f16x1+f32acc"}
+FORM_CODE = {"fft32": 0, "bf16": 1, "fp16": 2, "fp16x1": 4, "general": 3}
+FORM_DTYPE = {"fft32": "f32", "bf16": "bf16x3+f32acc", "fp16": "f16x3+f32acc", "fp16x1": "f16x1+f32acc",
+              "general": "f32"}
 # Vector issue peak: one wave64 VALU instruction per CU per cycle, 256 CUs at the 2.4 GHz peak
 # engine clock (FP32 work here issues no packed FP32 instructions, so this is also the FP32 roof).
 VALU_PEAK_GINST_S = 256 * 2.4
@@ -308,6 +309,8 @@ def main():
             launches, _ = wb.info()
             kms, klaunch = launches[0]["tuned_ms"], 1
         kavg = kms / klaunch / 1e3 if klaunch > 0 else None
+        if form == "general":
+            kavg = None   # its timed interval is the image-formation kernel alone, not the path
         return ms, kavg, cents[0].cpu().numpy().reshape(-1, 2)
 
     t0 = time.perf_counter()

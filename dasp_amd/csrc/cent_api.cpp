@@ -116,6 +116,7 @@ struct dasp_cent {
   dasp_cent_params p{};
   int nsubx = 0, nsubaps = 0, n = 0, F = 0, clip = 0, nimg = 0, ncen = 0;
   int psf = 0, padded = 0, pbf = 1, preBinOrig = 1, spotDim = 0;
+  int binB = 0;   // integer binning factor when the binning operator is the plain block sum
   int corrsize = 0, corrPatternSize = 0, npup = 0;
   int atmosPhaseType = 0, interpolate = 0, inputPhaseSize = 0;
   int curInteg = 0, form = DASP_CENT_FORM_FFT32;
@@ -357,6 +358,15 @@ static int upload_tables(dasp_cent *c) {
     }
   }
   if (upload(c->d_binA, A.data(), A.size() * 4)) return fail(DASP_CENT_EHIP, "upload of binning table failed");
+  // an integer factor gives the plain block sum (the general kernel then adds the blocks directly)
+  c->binB = 0;
+  if (Q % M == 0) {
+    const int B = Q / M;
+    bool block = true;
+    for (int I = 0; I < M && block; ++I)
+      for (int i = 0; i < Q && block; ++i) block = A[(size_t)I * Q + i] == (i / B == I ? 1.f : 0.f);
+    if (block) c->binB = B;
+  }
   // fit matrix for the effective fit size
   int fs = c->parabolicFit + c->gaussianFit;
   if (fs >= 0 && fs < 3) fs = 3;
@@ -478,6 +488,7 @@ static void fill_back(dasp_cent *c, BackArgs &b, int nframes) {
   b.otf = c->spotDim ? c->d_otf.as<float2>() : nullptr;
   b.otf_per_subap = c->spotDim == 4;
   b.binA = c->d_binA.as<float>();
+  b.binB = c->binB;
   b.scale = c->d_scale.as<float>();
   b.totsig = c->d_totsig.as<float>();
   b.sigv = c->d_sigv.as<float>();

@@ -14,15 +14,36 @@
 namespace dasp {
 
 constexpr int BT = 256;            // threads per workgroup
-constexpr int SIDE2 = kMaxSide * kMaxSide;
 
+// The workgroup's LDS, carved from dynamic shared memory sized for the handle's geometry (a
+// 32 x 32 image with a 16 x 16 detector needs about 11 KB, where room for the largest supported
+// side, 64, took 66 KB and held the kernel to two workgroups per CU).
 struct BackLds {
-  float h[SIDE2];                  // hll / convolved image / correlation image
-  float2 c[SIDE2];                 // complex work buffer; also sort and binning scratch
-  float img[SIDE2];                // detector image (nimg x nimg)
-  float red[BT];
-  int ired[BT];
-  float2 tw[kMaxSide / 2];
+  float *h;                        // hll / convolved image / correlation image
+  float2 *c;                       // complex work buffer; also sort and binning scratch
+  float *img;                      // detector image (nimg x nimg)
+  float *red;                      // BT
+  int *ired;                       // BT
+  float2 *tw;                      // kMaxSide / 2
+};
+
+// Sizes (elements) of the regions for one launch: h floats, c complex values, img floats.
+struct BackLdsSizes {
+  int h, c, img;
+  __host__ __device__ BackLdsSizes(const BackArgs &a) {
+    auto mx = [](int x, int y) { return x > y ? x : y; };
+    auto sq = [](int x) { return x * x; };
+    const int M2 = sq(a.nimg);
+    int n2 = 1;
+    while (n2 < M2) n2 <<= 1;
+    h = mx(mx(sq(a.padded), sq(a.psf)), mx(mx(sq(a.corrsize), M2), sq(a.clip)));
+    c = mx(mx(sq(a.psf), sq(a.corrsize)), (mx(mx(sq(a.padded), a.nimg * a.clip), n2) + 1) / 2);
+    img = M2;
+  }
+  __host__ __device__ size_t bytes() const {
+    return (size_t)c * sizeof(float2) + (size_t)(kMaxSide / 2) * sizeof(float2) + (size_t)h * sizeof(float) +
+           (size_t)img * sizeof(float) + (size_t)BT * (sizeof(float) + sizeof(int));
+  }
 };
 
 __device__ __forceinline__ float block_sum(float v, BackLds &L) {
@@ -277,7 +298,17 @@ __device__ float interp_cal(float x, int n, const float *xp, const float *fp) {
 }
 
 __global__ __launch_bounds__(BT) void backend_kernel(BackArgs a) {
-  __shared__ BackLds L;
+  extern __shared__ float2 back_lds[];
+  BackLds L;
+  {
+    const BackLdsSizes z(a);
+    L.c = back_lds;
+    L.tw = L.c + z.c;
+    L.h = reinterpret_cast<float *>(L.tw + kMaxSide / 2);
+    L.img = L.h + z.h;
+    L.red = L.img + z.img;
+    L.ired = reinterpret_cast<int *>(L.red + BT);
+  }
   const int tid = threadIdx.x;
   const long item = blockIdx.x;
   const int frame = (int)(item / a.nactive);
@@ -340,6 +371,20 @@ __global__ __launch_bounds__(BT) void backend_kernel(BackArgs a) {
     const int Q = a.clip;
     if (Q == M) {
       for (int q = tid; q < M2; q += BT) img[q] = L.h[q];
+    } else if (a.binB > 0) {
+      // the plain block sum, in the operator's order: each column's B rows, then the B columns
+      const int B = a.binB;
+      for (int q = tid; q < M2; q += BT) {
+        const int I = q / M, J = q % M;
+        const float *hb = L.h + (I * B) * Q + J * B;
+        float t = 0.f;
+        for (int dj = 0; dj < B; ++dj) {
+          float u = 0.f;
+          for (int di = 0; di < B; ++di) u += hb[di * Q + dj];
+          t += u;
+        }
+        img[q] = t;
+      }
     } else {
       float *tmp = reinterpret_cast<float *>(L.c);
       for (int q = tid; q < M * Q; q += BT) {
@@ -697,7 +742,17 @@ __global__ __launch_bounds__(BT) void backend_kernel(BackArgs a) {
 
 hipError_t launch_backend(const BackArgs &a, hipStream_t st) {
   if (a.nitems <= 0) return hipSuccess;
-  hipLaunchKernelGGL(backend_kernel, dim3((unsigned)a.nitems), dim3(BT), 0, st, a);
+  const size_t bytes = BackLdsSizes(a).bytes();
+  if (bytes > 64 * 1024) {   // the largest geometries (side 64) need more than the default limit
+    static bool raised = false;
+    if (!raised) {
+      const hipError_t e = hipFuncSetAttribute((const void *)backend_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                               (int)(160 * 1024));
+      if (e != hipSuccess) return e;
+      raised = true;
+    }
+  }
+  hipLaunchKernelGGL(backend_kernel, dim3((unsigned)a.nitems), dim3(BT), (unsigned)bytes, st, a);
   return hipGetLastError();
 }
 

@@ -83,8 +83,9 @@ struct BackArgs {
   // LGS spot kernel: full Hermitian spectrum (psf x psf complex) per subap or shared
   const float2 *otf;
   int otf_per_subap;
-  // binning operator (nimg x clip)
+  // binning operator (nimg x clip); binB > 0 when it is the plain binB x binB block sum
   const float *binA;
+  int binB;
   // normalisation
   const float *scale;    // per subap factor sig*frac/totsig (1 where totsig <= 0)
   const float *totsig;   // per subap theoretical total
