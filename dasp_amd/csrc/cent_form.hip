@@ -88,10 +88,21 @@ __global__ __launch_bounds__((64 * FormShape<NP, F>::W)) void form_hll_kernel(Fo
       const int o = (F - Q) >> 1;
       if (k >= o && k < o + Q) {
         float *dst = hll + (long)(Q - 1 - (k - o)) * Q;
-        static_for<0, F>([&](auto ll) {
-          constexpr int l = decltype(ll)::value;
-          if (l >= o && l < o + Q) dst[Q - 1 - (l - o)] = cabs2(X[l]);
-        });
+        if ((Q & 3) == 0 && (o & 3) == 0) {
+          // four rotated values per 16-byte store (the row's pixels l = o+4m .. o+4m+3 land at
+          // Q-4(m+1) .. Q-1-4m), a quarter of the store instructions of one value each
+          static_for<0, F / 4>([&](auto mm) {
+            constexpr int m = decltype(mm)::value, l = 4 * m;
+            if (l >= o && l < o + Q)
+              *reinterpret_cast<float4 *>(dst + Q - 4 - (l - o)) =
+                  make_float4(cabs2(X[l + 3]), cabs2(X[l + 2]), cabs2(X[l + 1]), cabs2(X[l]));
+          });
+        } else {
+          static_for<0, F>([&](auto ll) {
+            constexpr int l = decltype(ll)::value;
+            if (l >= o && l < o + Q) dst[Q - 1 - (l - o)] = cabs2(X[l]);
+          });
+        }
       }
     } else {
       const int rs = (Q - F) >> 1;
