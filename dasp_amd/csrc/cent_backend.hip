@@ -1,6 +1,6 @@
 // cent_backend.hip - per-subaperture image pipeline after image formation, general form.
 //
-// One workgroup (256 threads) per (frame, subaperture). The image lives in LDS; every option of
+// One workgroup (one wave) per (frame, subaperture). The image lives in LDS; every option of
 // centroidsFromPhase / centroidsFromImage (cmod/src/centmodule.c:1808-1961, :2000-2110) is
 // supported: pre-binning, LGS spot convolution, fractional binning, normalisation, sky,
 // optical binning, Poisson and read noise from pre-drawn normals, floor thresholds, brightest
@@ -10,10 +10,11 @@
 // common configurations; this one is the general path.
 #include "cent_internal.h"
 #include "noise.h"
+#include "wave_ops.h"
 
 namespace dasp {
 
-constexpr int BT = 256;            // threads per workgroup
+constexpr int BT = 64;             // threads per workgroup: one wave per item
 
 // The workgroup's LDS, carved from dynamic shared memory sized for the handle's geometry (a
 // 32 x 32 image with a 16 x 16 detector needs about 11 KB, where room for the largest supported
@@ -37,7 +38,12 @@ struct BackLdsSizes {
     int n2 = 1;
     while (n2 < M2) n2 <<= 1;
     h = mx(mx(sq(a.padded), sq(a.psf)), mx(mx(sq(a.corrsize), M2), sq(a.clip)));
-    c = mx(mx(sq(a.psf), sq(a.corrsize)), (mx(mx(sq(a.padded), a.nimg * a.clip), n2) + 1) / 2);
+    // the complex buffer only for the options that use it: spot convolution, FFT correlation,
+    // pre-binning, the separable (fractional) binning and the brightest-pixel sort
+    const bool sort = a.useBrightest != 0 || a.useBrightestArr != nullptr;
+    c = mx(mx(a.otf ? sq(a.psf) : 0, a.corrMode == 1 ? sq(a.corrsize) : 0),
+           (mx(mx(a.pbf > 1 ? sq(a.padded) : 0, (a.binB > 0 || a.clip == a.nimg) ? 0 : a.nimg * a.clip),
+               sort ? n2 : 0) + 1) / 2);
     img = M2;
   }
   __host__ __device__ size_t bytes() const {
@@ -46,29 +52,14 @@ struct BackLdsSizes {
   }
 };
 
-__device__ __forceinline__ float block_sum(float v, BackLds &L) {
-  L.red[threadIdx.x] = v;
-  __syncthreads();
-  for (int s = BT / 2; s > 0; s >>= 1) {
-    if ((int)threadIdx.x < s) L.red[threadIdx.x] += L.red[threadIdx.x + s];
-    __syncthreads();
-  }
-  float r = L.red[0];
-  __syncthreads();
-  return r;
+// Sums and maxima over the workgroup (one wave): cross-lane (DPP) reductions, the result in every
+// lane. Called by every lane of the wave.
+__device__ __forceinline__ float block_sum(float v, BackLds &) {
+  static_assert(BT == 64, "one wave per workgroup");
+  return group_sum<64>(v);
 }
 
-__device__ __forceinline__ float block_max(float v, BackLds &L) {
-  L.red[threadIdx.x] = v;
-  __syncthreads();
-  for (int s = BT / 2; s > 0; s >>= 1) {
-    if ((int)threadIdx.x < s) L.red[threadIdx.x] = fmaxf(L.red[threadIdx.x], L.red[threadIdx.x + s]);
-    __syncthreads();
-  }
-  float r = L.red[0];
-  __syncthreads();
-  return r;
-}
+__device__ __forceinline__ float block_max(float v, BackLds &) { return group_max<64>(v); }
 
 __device__ __forceinline__ int bitrev_n(int v, int bits) {
   return (int)(__brev((unsigned)v) >> (32 - bits));
