@@ -773,6 +773,7 @@ static int launch_phase(dasp_cent *c, const float *d_phs, long phs_stride, int n
     fa.npup = c->npup;
     fa.padded = c->padded;
     fa.hll = sc.hll;
+    fa.fast_trig = c->fastTrig ? 1 : 0;
     int trc = timing_mark(c, st);
     if (trc) return trc;
     HIPCHK(launch_form(fa, np, c->F, st));

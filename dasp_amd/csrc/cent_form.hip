@@ -4,8 +4,9 @@
 // Follows computeHll + addPowerSpecCentral (cmod/src/centmodule.c:430-509). Layout: a wave owns
 // SPW = 64/NP subapertures; lane (s, j) loads column j of subaperture s with coalesced reads of
 // the phase rows, transforms it along i in registers (NP-point FFTs, the zero half of the F-point
-// input is never touched), the wave transposes through LDS once, and lane (s, m) transforms rows
-// k = m*R .. m*R+R-1 along j, again in registers.
+// input is never touched), the wave transposes through LDS in R = F/NP phases (phase r stages the
+// rows k = q*R + r, so that NP x NP values per subaperture sit in LDS at a time), and lane (s, m)
+// transforms row k = m*R + r along j, again in registers.
 #include "cent_internal.h"
 #include "fft_reg.h"
 
@@ -16,12 +17,12 @@ struct FormShape {
   static constexpr int SPW = 64 / NP;          // subapertures per wave
   static constexpr int R = F / NP;             // rows per lane in the second pass
   static constexpr int TS = NP + 1;            // transpose row stride (complex), padded
-  static constexpr int WAVE_LDS = SPW * F * TS * (int)sizeof(cf);
+  static constexpr int WAVE_LDS = SPW * NP * TS * (int)sizeof(cf);
   static constexpr int W = (65536 / WAVE_LDS) >= 4 ? 4 : ((65536 / WAVE_LDS) >= 2 ? 2 : 1);
 };
 
 // Column transform: lane holds E[0..NP-1][j] and gets Z[k][j], k < F.
-template <int NP, int F>
+template <int NP, int F, bool FAST>
 __device__ __forceinline__ void load_column(const float *__restrict__ ph, const float *__restrict__ am,
                                             const float *__restrict__ pu, const float *__restrict__ tilt,
                                             int n, int npup, int j, bool valid, cf (&x)[NP]) {
@@ -32,7 +33,7 @@ __device__ __forceinline__ void load_column(const float *__restrict__ ph, const 
       const float phi = ph[(long)i * npup + j];
       const float th = phi - tilt[i * n + j];   // float subtraction, as the reference (:484)
       float sn, cs;
-      sincosf(th, &sn, &cs);
+      phase_sincos<FAST>(th, &sn, &cs);
       float p = pu[i * n + j];
       if (am) p *= am[(long)i * npup + j];
       v = {p * cs, p * sn};
@@ -41,11 +42,11 @@ __device__ __forceinline__ void load_column(const float *__restrict__ ph, const 
   });
 }
 
-template <int NP, int F>
+template <int NP, int F, bool FAST>
 __global__ __launch_bounds__((64 * FormShape<NP, F>::W)) void form_hll_kernel(FormArgs a) {
   using S = FormShape<NP, F>;
   constexpr int SPW = S::SPW, R = S::R, TS = S::TS, W = S::W;
-  __shared__ cf tbuf[W][SPW * F * TS];
+  __shared__ cf tbuf[W][SPW * NP * TS];
   const int lane = threadIdx.x & 63;
   const int w = threadIdx.x >> 6;
   const int sl = lane / NP;
@@ -65,22 +66,26 @@ __global__ __launch_bounds__((64 * FormShape<NP, F>::W)) void form_hll_kernel(Fo
   const float *pu = a.pup + (long)s * n * n;
 
   cf x[NP];
-  load_column<NP, F>(ph, am, pu, a.tilt, n, a.npup, j, valid, x);
+  load_column<NP, F, FAST>(ph, am, pu, a.tilt, n, a.npup, j, valid, x);
   cf z[F];
   fft_fwd_zp<NP, F>(x, z);
-  cf *tb = &tbuf[w][sl * F * TS];
-  static_for<0, F>([&](auto kk) {
-    constexpr int k = decltype(kk)::value;
-    tb[k * TS + j] = z[k];
-  });
-  __syncthreads();
+  cf *tb = &tbuf[w][sl * NP * TS];
 
   const int Q = a.padded;
   float *hll = a.hll + item * (long)Q * Q;
   static_for<0, R>([&](auto rr) {
-    const int k = j * R + decltype(rr)::value;
+    constexpr int r = decltype(rr)::value;
+    // phase r: rows k = q*R + r of the column spectra through the wave's LDS
+    static_for<0, NP>([&](auto qq) { tb[decltype(qq)::value * TS + j] = z[decltype(qq)::value * R + r]; });
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    const int k = j * R + r;
     cf row[NP];
-    static_for<0, NP>([&](auto jj) { row[decltype(jj)::value] = tb[k * TS + decltype(jj)::value]; });
+    static_for<0, NP>([&](auto jj) { row[decltype(jj)::value] = tb[j * TS + decltype(jj)::value]; });
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     cf X[F];
     fft_fwd_zp<NP, F>(row, X);
     if (!valid) return;
@@ -121,7 +126,8 @@ static hipError_t launch_form_t(const FormArgs &a, hipStream_t st) {
   const long per_block = (long)S::W * S::SPW;
   const long blocks = (a.nitems + per_block - 1) / per_block;
   if (blocks <= 0) return hipSuccess;
-  hipLaunchKernelGGL((form_hll_kernel<NP, F>), dim3((unsigned)blocks), dim3(64 * S::W), 0, st, a);
+  if (a.fast_trig) hipLaunchKernelGGL((form_hll_kernel<NP, F, true>), dim3((unsigned)blocks), dim3(64 * S::W), 0, st, a);
+  else hipLaunchKernelGGL((form_hll_kernel<NP, F, false>), dim3((unsigned)blocks), dim3(64 * S::W), 0, st, a);
   return hipGetLastError();
 }
 

@@ -71,6 +71,7 @@ struct FormArgs {
   int nsubx, n, npup;
   int padded;            // paddedsize: side of hll (clip, or psfsize with a spot kernel)
   float *hll;            // (nitems, padded, padded)
+  int fast_trig;         // the phase factor's sin/cos as in FusedArgs::fast_trig
 };
 
 // Everything after image formation, one (frame, subaperture) item per workgroup.

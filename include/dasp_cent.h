@@ -170,10 +170,10 @@ int dasp_cent_set_form(dasp_cent_t *h, int form);
  * DASP_CENT_EINVAL when threshType is not 0. */
 int dasp_cent_set_relative_floor(dasp_cent_t *h, int enable);
 
-/* Phase factor exp(i(phi - t)) in the fused and DFT-GEMM kernels: 1 (default) = the argument
- * reduced to revolutions, r = x*c_hi - k + x*c_lo with c_hi + c_lo = 1/(2 pi) (reduction error at
- * most 2.6e-7 rad for |x| <= 400 rad), then the hardware sine/cosine (v_sin_f32/v_cos_f32);
- * 0 = sincosf (max abs error 7e-8). The general kernels use sincosf. */
+/* Phase factor exp(i(phi - t)) in the image kernels (fused, DFT-GEMM and the general path's image
+ * formation): 1 (default) = the argument reduced to revolutions, r = x*c_hi - k + x*c_lo with
+ * c_hi + c_lo = 1/(2 pi) (reduction error at most 2.6e-7 rad for |x| <= 400 rad), then the hardware
+ * sine/cosine (v_sin_f32/v_cos_f32); 0 = sincosf (max abs error 7e-8). */
 int dasp_cent_set_fast_trig(dasp_cent_t *h, int enable);
 
 /* Launch grids of the single-WFS kernels: image_grid for the image kernel (fused or DFT-GEMM
