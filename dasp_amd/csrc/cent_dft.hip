@@ -402,14 +402,22 @@ void dft16_pipe_kernel(FusedArgs a_) {
     f = (int)q;
     p = (int)(pr - q * pp);
   };
-  auto load_pair = [&](long pr, int f, int p, float (&ph8)[8], float (&pu8)[8]) {
-    const int u = c >> 4;
-    const bool vA = pr < npairs, vB = vA && 2 * p + 1 < cnt;
+  // a pair's sub-aperture indices and window offsets (scalar reads), fetched one iteration before
+  // its loads are issued so that the loads do not wait for them
+  struct PairIdx {
+    int rA, rB, oA, oB;
+  };
+  auto fetch_idx = [&](int p) {
     // unconditional in-slice reads (p < pp), selected afterwards: no branch in the loop body;
     // window offsets from the per-sub-aperture table (no division)
     const int iA = s0 + 2 * p, iB = s0 + min(2 * p + 1, cnt - 1);
-    const int rA = scalar_read(a.active, iA), rB = scalar_read(a.active, iB);
-    const int oA = scalar_read(a.winoff, iA), oB = scalar_read(a.winoff, iB);
+    return PairIdx{scalar_read(a.active, iA), scalar_read(a.active, iB), scalar_read(a.winoff, iA),
+                   scalar_read(a.winoff, iB)};
+  };
+  auto load_pair = [&](long pr, int f, int p, const PairIdx &ix, float (&ph8)[8], float (&pu8)[8]) {
+    const int u = c >> 4;
+    const bool vA = pr < npairs, vB = vA && 2 * p + 1 < cnt;
+    const int rA = ix.rA, rB = ix.rB, oA = ix.oA, oB = ix.oB;
     // Item A's phase window and pupil are wave-uniform bases; the lanes of item B add B's
     // distance from A (non-negative: active indices and window offsets ascend), so each load is
     // a scalar base plus one 32-bit lane offset, with no 64-bit address arithmetic per load.
@@ -461,7 +469,7 @@ void dft16_pipe_kernel(FusedArgs a_) {
   int cf, cp;
   pair_ref(wave_id, cf, cp);
   float ph8[8], pu8[8];
-  load_pair(wave_id, cf, cp, ph8, pu8);
+  load_pair(wave_id, cf, cp, fetch_idx(cp), ph8, pu8);
   f16v zr = {}, zi = {};
   {
     COp<T8, COMP> A1;
@@ -470,7 +478,10 @@ void dft16_pipe_kernel(FusedArgs a_) {
   }
   int nf = cf, np = cp;
   advance(nf, np);
-  load_pair(wave_id + nwaves, nf, np, ph8, pu8);
+  load_pair(wave_id + nwaves, nf, np, fetch_idx(np), ph8, pu8);
+  int xf = nf, xp = np;   // the pair two ahead of the current one: its indices are fetched early
+  advance(xf, xp);
+  PairIdx xidx = fetch_idx(xp);
   for (long q = wave_id; q < npairs; q += nwaves) {
     // Three scheduling regions (nothing crosses a sched_barrier), each pairing matrix-core work
     // with independent vector work: (1) stage 2 of pair q with the phase factor of the pair
@@ -564,7 +575,9 @@ void dft16_pipe_kernel(FusedArgs a_) {
     cf = nf;
     cp = np;
     advance(nf, np);
-    load_pair(q + 2 * nwaves, nf, np, ph8, pu8);   // two pairs ahead
+    load_pair(q + 2 * nwaves, nf, np, xidx, ph8, pu8);   // two pairs ahead
+    advance(xf, xp);
+    xidx = fetch_idx(xp);                                  // three pairs ahead
     // ---- stores ----
     const int nvalid = 2 * p + 1 < cnt ? 2 : 1;
     if (!BINS && a.bimg) {   // bins_only holds only without an image buffer
