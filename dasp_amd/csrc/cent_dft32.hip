@@ -60,8 +60,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(COMP || !FU
   }
   __syncthreads();
   // XCD-aware item order: workgroup b runs on XCD b % 8, and XCD x takes the x-th eighth of the
-  // active sub-apertures for every frame, so that its L2 keeps that slice's pupil. Within an XCD
-  // the slice's items (frame-major) are taken with a grid stride.
+  // active sub-apertures for every frame. Within an XCD the slice's items are taken with a grid
+  // stride in sub-aperture-major order (all frames of one sub-aperture, then the next), so that the
+  // XCD's waves work on one or two sub-apertures at a time and their pupils stay in L2 (in
+  // frame-major order the slice's whole pupil, 4 KB per sub-aperture at n = 32, competed with the
+  // streamed phase and was partly read again from HBM for every frame).
   const int xcd = blockIdx.x % kXcds;
   const int s0 = (int)((long)a.nactive * xcd / kXcds);
   const int cnt = (int)((long)a.nactive * (xcd + 1) / kXcds) - s0;
@@ -71,12 +74,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(COMP || !FU
   zero_inactive_slopes(a);
   if (wave_id >= nx) return;   // wave-uniform; the workgroup has no barrier after this point
   const int n = a.n;
-  // local item q = f * cnt + r (sub-aperture s0 + r of frame f), q advancing by nwaves: one
+  // local item q = r * nfr + f (sub-aperture s0 + r of frame f), q advancing by nwaves: one
   // scalar division per wave, then a fixed step
+  const int nfr = (int)(a.nitems / a.nactive);
   auto split_item = [&](long q, int &f, int &r) {
-    const long fq = q / cnt;
-    f = (int)fq;
-    r = (int)(q - fq * cnt);
+    const long rq = q / nfr;
+    r = (int)rq;
+    f = (int)(q - rq * nfr);
   };
   int stepF, stepR;
   split_item(nwaves, stepF, stepR);
@@ -119,9 +123,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(COMP || !FU
   load_item(wave_id, cf, cidx, cph, cpu);
   for (long q = wave_id; q < nx; q += nwaves) {
     int nf = cf + stepF, nidx = cidx + stepR;
-    if (nidx >= cnt) {
-      nidx -= cnt;
-      ++nf;
+    if (nf >= nfr) {
+      nf -= nfr;
+      ++nidx;
     }
     const int frame = cf, s = scalar_read(a.active, s0 + cidx);
     // ---- E = pup exp(i (phi - t)) and its hi/lo operands ----

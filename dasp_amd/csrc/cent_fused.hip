@@ -17,8 +17,12 @@ namespace dasp {
 // workgroup b runs on XCD b % 8 (round-robin dispatch), and XCD x owns the x-th eighth of the
 // subaperture groups for every frame, so each XCD's L2 keeps its slice of the pupil (and of any
 // per-subaperture table) across frames instead of every frame re-reading the whole pupil. Inside
-// an XCD the groups are taken frame-major with a grid stride, and each wave loads its next
-// group's phase and pupil while it transforms the current one.
+// an XCD the groups are taken with a grid stride, frame-major, or for n = 32 (4 KB of pupil per
+// sub-aperture, where the slice's pupil competes with the streamed phase for L2)
+// sub-aperture-group-major: every frame of a group, then the next group, so that the XCD's waves
+// work on a few groups at a time (1 % faster at n32/nfft64, 1 % slower at n16/nfft32 with noise,
+// same-box A/B). Each wave loads its next group's phase and pupil while it transforms the current
+// one.
 constexpr int kXcds = 8;
 
 template <int NP, int F, int B, bool FAST, bool SPOT>
@@ -45,11 +49,15 @@ void fused_cog_kernel(FusedArgs a_) {
   long q = (long)(blockIdx.x / kXcds) * S::WAVES + w;
   zero_inactive_slopes(a);
   if (q >= ngx) return;           // wave-uniform
-  // group q = f * nsgx + g (group sg0 + g of frame f); q advances by a fixed stride, so frame
-  // and group advance by fixed steps: one scalar division per wave
-  const int nsgi = (int)nsgx;
-  int fcur = (int)(q / nsgx), gcur = (int)(q - (long)fcur * nsgx);
-  const int stepF = (int)(stride / nsgx), stepG = (int)(stride - (long)stepF * nsgx);
+  // group q = g * nframes + f (sub-aperture-group-major) or q = f * nsgx + g (frame-major) for
+  // group sg0 + g of frame f; q advances by a fixed stride, so frame and group advance by fixed
+  // steps: one scalar division per wave
+  constexpr bool kGroupMajor = NP >= 32;
+  const long inner = kGroupMajor ? nframes : nsgx;   // the faster-moving index's range
+  const int ninner = (int)inner;
+  const int qo = (int)(q / inner), qi = (int)(q - (long)qo * inner);
+  int gcur = kGroupMajor ? qo : qi, fcur = kGroupMajor ? qi : qo;
+  const int stepO = (int)(stride / inner), stepI = (int)(stride - (long)stepO * inner);
   auto item_of = [&](long qq, int f, int g) -> ItemRef {   // wave-uniform arguments
     return item_at(a, qq < ngx ? f : -1, (int)((sg0 + g) * SPW) + sl);
   };
@@ -58,11 +66,12 @@ void fused_cog_kernel(FusedArgs a_) {
   ItemRef it = item_of(q, fcur, gcur);
   load_item<NP, true>(a, it, j, cur, fcur);
   for (; q < ngx; q += stride) {
-    int nf = fcur + stepF, ng = gcur + stepG;
-    if (ng >= nsgi) {
-      ng -= nsgi;
-      ++nf;
+    int ni = (kGroupMajor ? fcur : gcur) + stepI, no = (kGroupMajor ? gcur : fcur) + stepO;
+    if (ni >= ninner) {
+      ni -= ninner;
+      ++no;
     }
+    const int nf = kGroupMajor ? ni : no, ng = kGroupMajor ? no : ni;
     PhaseCol<NP> nxt;
     const bool more = q + stride < ngx;
     const ItemRef itn = item_of(q + stride, nf, ng);
