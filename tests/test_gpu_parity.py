@@ -140,6 +140,28 @@ def test_correlation_parabolic_fit():
     c.free()
 
 
+def test_general_path_largest_geometry():
+    """The general path at the largest supported sides: nimg 64 with an FFT correlation of 64 x 64
+    (the per-item LDS region is sized from the geometry; this one needs more than 64 KB, the
+    default limit of dynamically sized LDS), against the FP64 oracle."""
+    say("case general path nimg 64, corrsize 64")
+    nsubx, n, nfft, nimg = 2, 32, 64, 64
+    wgeo = W.WFS("big", nsubx, n, nfft, nimg, 2.0, 0.0, 0.2, 25.0)
+    ref = W.reference_images(wgeo)
+    c = Case(nsubx, n, nfft, nimg, obs=0.0, correlationCentroiding=1, corrPattern=ref, parabolicFit=5,
+             phs=wgeo.phase() * 0.5, form=3)
+    oc, ob, ocorr = c.run_both()
+    act = _active(c)
+    assert act.sum() >= 1
+    d = report("general nimg 64", c.cents, oc, act)
+    assert d.max() < 1e-4
+    b = c.bimg.reshape(-1, nimg, nimg)[act]
+    np.testing.assert_allclose(b, ob[act], rtol=0, atol=2e-5 * ob[act].max())
+    ci = c.corrimg.reshape(-1, nimg, nimg)[act]
+    np.testing.assert_allclose(ci, ocorr[act], rtol=0, atol=1e-5 * np.abs(ocorr[act]).max())
+    c.free()
+
+
 @pytest.mark.parametrize("fit", [0, 3], ids=["centred_cog", "parabolic3"])
 @pytest.mark.parametrize("nsubx", [5, 7])
 def test_correlation_padded_odd_count(nsubx, fit):
