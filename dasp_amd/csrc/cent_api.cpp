@@ -539,7 +539,8 @@ static bool need_pois(dasp_cent *c) { return c->addPoisson == 1 && c->calsource 
 static bool need_read(dasp_cent *c) { return c->readnoise != 0.f && c->calsource == 0; }
 
 // The fused kernel covers: no spot kernel, clip == fftsize, integer binning of a shape it was
-// instantiated for, phase only, plain or weighted CoG without calibration or pixel selection.
+// instantiated for, phase only, plain or weighted CoG without calibration; the brightest-pixel
+// selection in the FP32 kernels only.
 static bool fused_spot(dasp_cent *c) {
   return c->spotDim != 0 && c->psf == c->F && c->padded == c->F && c->pbf == 1 && c->F == 2 * pad_pow2(c->n) &&
          c->d_otfk.p != nullptr;
@@ -560,17 +561,21 @@ static bool corr_stage_ok(dasp_cent *c) {
 static bool fused_options(dasp_cent *c) {
   return (c->spotDim == 0 || fused_spot(c)) && c->clip == c->F && c->atmosPhaseType == 0 && c->opticalBinning == 0 &&
          c->imageOnly == 0 && corr_stage_ok(c) &&
-         !c->haveCalData && !c->haveCalCoeff && (c->calsource != 0 || (c->useBrightest == 0 && !c->haveUBArr));
+         !c->haveCalData && !c->haveCalCoeff;
 }
 
-// The DFT-GEMM kernels cover the fused options for n = 16 / fftsize 32 and n in 17..32 /
+// selectBrightestPixels runs only without calsource (cmod/src/centmodule.c:1883-1884).
+static bool brightest_on(dasp_cent *c) { return c->calsource == 0 && (c->useBrightest != 0 || c->haveUBArr); }
+
+// The DFT-GEMM kernels cover the fused options (no brightest-pixel selection) for n = 16 / fftsize 32 and n in 17..32 /
 // fftsize 64, nimg 16.
 static bool is_dft_form(int form) {
   return form == DASP_CENT_FORM_DFT_FP16 || form == DASP_CENT_FORM_DFT_BF16 || form == DASP_CENT_FORM_DFT_FP16_1;
 }
 
 static bool use_dft(dasp_cent *c) {
-  return is_dft_form(c->form) && fused_options(c) && c->spotDim == 0 && dft_supported(c->n, c->F, c->nimg);
+  return is_dft_form(c->form) && fused_options(c) && !brightest_on(c) && c->spotDim == 0 &&
+         dft_supported(c->n, c->F, c->nimg);
 }
 
 static bool use_fused(dasp_cent *c) {
@@ -619,7 +624,10 @@ static void fused_args(dasp_cent *c, const float *d_phs, long phs_stride, int nf
   fa.bimg = d_bimg;
   fa.bimg_stride = (long)c->nsubaps * c->nimg * c->nimg;
   fa.slow_detector = (d_zpois && need_pois(c)) || (d_zread && need_read(c)) || c->haveSkyArr ||
-                     kthresh(c) == 2 || (c->pxlPower != 1.f && c->pxlPower != 0.f) || c->haveCentWeight;
+                     kthresh(c) == 2 || (c->pxlPower != 1.f && c->pxlPower != 0.f) || c->haveCentWeight ||
+                     brightest_on(c);
+  fa.useBrightest = brightest_on(c) && !c->haveUBArr ? c->useBrightest : 0;
+  fa.useBrightestArr = brightest_on(c) && c->haveUBArr ? c->d_ubArr.as<int>() : nullptr;
   {
     // the fast detector's pixel is scale * bin + (sky + readbg), then the threshold of type tt
     const int tt = fa.threshType;
@@ -1435,9 +1443,10 @@ int dasp_cent_set_form(dasp_cent_t *c, int form) {
   c->gen++;
   if (form < DASP_CENT_FORM_FFT32 || form > DASP_CENT_FORM_DFT_FP16_1) return fail(DASP_CENT_EINVAL, "unknown form %d", form);
   if (is_dft_form(form) &&
-      !(fused_options(c) && c->spotDim == 0 && dft_supported(c->n, c->F, c->nimg)))
+      !(fused_options(c) && !brightest_on(c) && c->spotDim == 0 && dft_supported(c->n, c->F, c->nimg)))
     return fail(DASP_CENT_EUNSUPPORTED, "the DFT-GEMM form covers n=16 with fftsize=clipsize=32 and n=17..32 with "
-                                        "fftsize=clipsize=64, nimg=16, the fused options, no spot kernel, phase only");
+                                        "fftsize=clipsize=64, nimg=16, the fused options without brightest-pixel selection, "
+                                        "no spot kernel, phase only");
   c->form = form;
   return 0;
 }

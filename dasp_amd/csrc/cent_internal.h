@@ -208,6 +208,11 @@ struct FusedArgs {
   // image returned, nothing added, a threshold that changes nothing, the whole image as CoG
   // window, every scale > 0): the CoG may then be taken on the bins themselves
   int bins_only;
+  // the brightest-pixel selection (selectBrightestPixels): a count for every sub-aperture, or one
+  // per sub-aperture (useBrightestArr, non-null), 0 for none; negative counts subtract the largest
+  // value below the cut. Both off when calsource is set.
+  int useBrightest;
+  const int *useBrightestArr;
 };
 
 // Slopes of the inactive sub-apertures set to zero for every frame (centroidsFromPhase gives
@@ -286,8 +291,8 @@ struct BatchDesc {
 
 // Kernel argument blocks travel by value between separately compiled objects: a layout change
 // must come with a full rebuild (dasp_amd/build.py compiles every source each time).
-static_assert(sizeof(FusedArgs) == 272, "FusedArgs layout changed: update this pin and rebuild all objects");
-static_assert(sizeof(BatchDesc) == 280, "BatchDesc layout changed: update this pin and rebuild all objects");
+static_assert(sizeof(FusedArgs) == 288, "FusedArgs layout changed: update this pin and rebuild all objects");
+static_assert(sizeof(BatchDesc) == 296, "BatchDesc layout changed: update this pin and rebuild all objects");
 
 struct BatchArgs {
   const BatchDesc *desc; // ascending group0, desc[0].group0 == 0

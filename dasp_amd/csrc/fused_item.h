@@ -439,12 +439,83 @@ __device__ __forceinline__ void fused_item(const FusedArgs &a, const ItemRef &it
       mx = group_max<NP>(mx);
       if (a.floor > 0.f && a.floor <= 1.f) thr = a.floor * fmaxf(mx, 0.f);
     }
+    int tt = threshType;   // the threshold the pixel loop below applies
+    {
+      // brightest-pixel selection (cmod/src/centmodule.c:1544-1581), after readCCD's threshold:
+      // the ub-th largest of the M2 values (with repeats) is the cut, found by a bitwise search
+      // on order-preserving integer keys that counts the group's values at or above each
+      // candidate; the selection is group-uniform (one item per lane group)
+      int ub = valid ? (a.useBrightestArr ? a.useBrightestArr[s] : a.useBrightest) : 0;
+      const bool bsub = ub < 0;
+      if (bsub) ub = -ub;
+      if (ub > 0 && ub < M2) {
+        // keys in registers up to 16 pixels a lane; above that in place of the values in LDS
+        // (the registers would lower the occupancy of the kernel as a whole), turned back into
+        // values when the cut is applied
+        constexpr bool KREG = PX <= 16;
+        unsigned kreg[KREG ? PX : 1];
+        unsigned *klds = reinterpret_cast<unsigned *>(px);
+        auto key_of = [](float v) {
+          const unsigned u = __builtin_bit_cast(unsigned, v);
+          return (u >> 31) ? ~u : (u | 0x80000000u);
+        };
+        auto value_of = [](unsigned k) { return __builtin_bit_cast(float, (k >> 31) ? (k & 0x7fffffffu) : ~k); };
+        static_for<0, PX>([&](auto qq) {
+          constexpr int q = decltype(qq)::value;
+          float v = px[q];
+          if (tt == 1) v = v < thr ? 0.f : v;
+          else if (tt == 0 || tt == 2) v = v < thr ? 0.f : v - thr;
+          v = v == 0.f ? 0.f : v;   // -0 as +0: keys then order and tie as the values compare
+          // key 0 (below every value's key) for lanes that hold no pixels of their own
+          const unsigned k = owner ? key_of(v) : 0u;
+          if constexpr (KREG) {
+            kreg[q] = k;
+            px[q] = v;
+          } else {
+            klds[q] = k;
+          }
+        });
+        auto key = [&](auto qq) -> unsigned {
+          if constexpr (KREG) return kreg[decltype(qq)::value];
+          else return klds[decltype(qq)::value];
+        };
+        tt = -1;
+        unsigned cut = 0u;
+        const float need = (float)ub;
+#pragma unroll 1
+        for (int b = 31; b >= 0; --b) {
+          const unsigned cand = cut | (1u << b);
+          int cnt = 0;
+          static_for<0, PX>([&](auto qq) { cnt += key(qq) >= cand ? 1 : 0; });
+          if (group_sum<NP>((float)cnt) >= need) cut = cand;   // counts below 2^24: exact
+        }
+        const float cv = value_of(cut);
+        float sub = 0.f;
+        if (bsub) {
+          // the largest value below the cut, 0 when there is none
+          float lo = -INFINITY;
+          static_for<0, PX>([&](auto qq) {
+            const unsigned k = key(qq);
+            if (k != 0u && k < cut) lo = fmaxf(lo, value_of(k));
+          });
+          lo = group_max<NP>(lo);
+          sub = lo == -INFINITY ? 0.f : lo;
+        }
+        static_for<0, PX>([&](auto qq) {
+          constexpr int q = decltype(qq)::value;
+          float v;
+          if constexpr (KREG) v = px[q];
+          else v = value_of(klds[q]);
+          px[q] = v < cv ? 0.f : v - sub;
+        });
+      }
+    }
     float *bout = a.bimg ? a.bimg + (long)frame * a.bimg_stride + pix0 : nullptr;
     const float *wt = a.centWeight ? a.centWeight + (a.centWeightDim == 4 ? pix0 : 0) : nullptr;
     if (pxlPower == 1.f && !wt) {
       // no pixel power and no weights (the usual noisy detector): the threshold by selects and
       // the window by a per-lane row test and compile-time columns, unrolled
-      const bool t1 = threshType == 1, t02 = threshType == 0 || threshType == 2;
+      const bool t1 = tt == 1, t02 = tt == 0 || tt == 2;
       const float tsub = t02 ? thr : 0.f;
       const bool thr_on = t1 || t02;
       static_for<0, PX>([&](auto qq) {
@@ -464,8 +535,8 @@ __device__ __forceinline__ void fused_item(const FusedArgs &a, const ItemRef &it
     for (int q = 0; q < PX; ++q) {
       const int I = det_row(q / NIMG), jb = q % NIMG;
       float v = px[q];
-      if (threshType == 1) v = v < thr ? 0.f : v;
-      else if (threshType == 0 || threshType == 2) v = v < thr ? 0.f : v - thr;
+      if (tt == 1) v = v < thr ? 0.f : v;
+      else if (tt == 0 || tt == 2) v = v < thr ? 0.f : v - thr;
       if (pxlPower == 2.f) v = v * v;
       else if (pxlPower != 1.f && pxlPower != 0.f) v = powf(v, pxlPower);
       if (owner) {

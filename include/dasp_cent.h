@@ -156,8 +156,10 @@ int dasp_cent_testcorrelation(const float *img, float *corr, int n);
 /* ---- extensions (no counterpart in the reference) ---------------------------------------- */
 
 /* Select the compute form (DASP_CENT_FORM_*). The DFT-GEMM forms cover nimg = 16 with n = 16 /
- * fftsize = clipsize = 32 and n = 17..32 / fftsize = clipsize = 64, the fused kernels' options and
- * no spot kernel; other geometries return DASP_CENT_EUNSUPPORTED. The single-product form
+ * fftsize = clipsize = 32 and n = 17..32 / fftsize = clipsize = 64, the fused kernels' options
+ * without brightest-pixel selection, and no spot kernel; other set-ups return
+ * DASP_CENT_EUNSUPPORTED (a count set later runs the handle on the fused FP32 kernel until it is
+ * cleared). The single-product form
  * (DASP_CENT_FORM_DFT_FP16_1) rounds each operand once to FP16 (no hi/lo compensation), also with
  * the detector options outside the inline detector (noise, sky array, relative floor, power,
  * weights). */

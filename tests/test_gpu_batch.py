@@ -1,7 +1,7 @@
 """GPU: the multi-WFS step (persistent work-queue launches + step graph) against per-WFS device
 runs of the same handles, and against the FP64 oracle.
 
-WFS of uneven nsubx/n/fftsize/nimg share one step; one WFS takes the general path (LGS spot
+WFS of uneven nsubx/n/fftsize/nimg and detector options share one step; one WFS takes the general path (LGS spot
 kernel) inside the same graph. The step graph is replayed with new phase written in place
 between replays, each replay checked against an eager per-WFS run."""
 import numpy as np
@@ -19,6 +19,8 @@ MIX = [
     ("ngs4_n32_b8", 4, 32, 64, 8, {}),
     ("wfs10_noise", 10, 16, 32, 16, dict(sig=1000.0, readnoise=0.5, addPoisson=1, noiseFloor=0.1, threshType=0, relativeFloor=True)),
     ("wfs6_n16_b4", 6, 16, 32, 8, {}),
+    # brightest-pixel selection (subtracting the next value) inside the fused launch
+    ("wfs8_brightest", 8, 8, 16, 8, dict(sig=100.0, useBrightest=-12)),
 ]
 
 

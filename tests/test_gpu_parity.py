@@ -122,6 +122,109 @@ def test_general_detector_every_fused_shape(name, kind):
     c.free()
 
 
+def _brightest_counts(S, M2, seed):
+    """Per-sub-aperture counts of every kind: both signs, none (0), the whole image (M2: no
+    selection), one pixel, all but one, random in between."""
+    ub = np.random.default_rng(seed).integers(-(M2 // 2), M2 // 2, S)
+    ub[:6] = [M2, -(M2 - 1), 1, -1, 0, M2 - 1]
+    return ub.astype(np.int32)
+
+
+def _brightest_pair(nsubx, n, nfft, nimg, ub, **kw):
+    """The same images without (a count of nimg^2 everywhere: the same detector code, nothing
+    selected) and with the selection; returns (case with selection, oracle slopes, oracle images,
+    the engine's images before selection)."""
+    S, M2 = nsubx * nsubx, nimg * nimg
+    c0 = Case(nsubx, n, nfft, nimg, useBrightest=np.full(S, M2, np.int32), form=0, **kw)
+    c0.run_both()
+    b0 = c0.bimg.reshape(S, nimg, nimg).copy()
+    c0.free()
+    c = Case(nsubx, n, nfft, nimg, useBrightest=ub, form=0, **kw)
+    oc, ob, _ = c.run_both()
+    return c, oc, ob, b0
+
+
+def _check_selection(c, b0, ub):
+    """The engine's selected images equal selectBrightestPixels of its own images before
+    selection, bit for bit (FP32 subtraction of the next value)."""
+    S = c.nsubx * c.nsubx
+    ubs = np.broadcast_to(np.asarray(ub), (S,))
+    act = _active(c)
+    want = np.stack([O.select_brightest(b0[s], int(ubs[s])).astype(np.float32) for s in range(S)])
+    got = c.bimg.reshape(S, c.nimg, c.nimg)
+    bad = (got != want).any(axis=(1, 2)) & act
+    say("selection differing from the oracle's on the engine's own images: %d sub-apertures" % bad.sum())
+    assert not bad.any()
+    cut_ties = 0
+    for s in np.nonzero(act)[0]:
+        u = abs(int(ubs[s]))
+        x = np.sort(b0[s].ravel())[::-1]
+        if 0 < u < x.size and x[u - 1] == x[u]:
+            cut_ties += 1
+    say("sub-apertures with a tie at the cut: %d" % cut_ties)
+    return cut_ties
+
+
+@pytest.mark.parametrize("name", sorted(FUSED_SHAPES))
+def test_brightest_every_fused_shape(name):
+    """selectBrightestPixels (cmod/src/centmodule.c:1544-1581) in the fused kernels' general
+    detector on every shape (keys in registers up to 16 pixels a lane, in LDS above), with the
+    per-sub-aperture count array and readCCD's type-1 floor before it (ties at zero wherever a
+    count reaches into the zeroed pixels). The selection must equal the oracle's on the engine's
+    own images before selection, bit for bit; against the FP64 oracle end to end, the
+    sub-apertures whose cut sits between two values closer than the FP32 image error are left out."""
+    nsubx, n, nfft, nimg = FUSED_SHAPES[name]
+    say("case brightest %s" % name)
+    S, M2 = nsubx * nsubx, nimg * nimg
+    w = W.WFS(name, nsubx, n, nfft, nimg, 4.2, 0.2, 0.15, 25.0)
+    ub = _brightest_counts(S, M2, 4)
+    c, oc, ob, b0 = _brightest_pair(nsubx, n, nfft, nimg, ub, obs=0.2, sig=200.0, noiseFloor=0.05 * 200.0 / M2,
+                                    threshType=1, phs=w.phase())
+    _check_selection(c, b0, ub)
+    import copy
+    p0 = copy.copy(c.params)
+    p0.useBrightest, p0.useBrightestArr = 0, None
+    _, ob0, _ = O.run_phase(p0, c.phs)
+    act = _active(c)
+    err = np.abs(b0 - ob0)[act].max()            # the FP32 image error before selection
+    fragile = np.zeros(S, bool)
+    for s in range(S):
+        u = abs(int(ub[s]))
+        x = np.sort(ob0[s].ravel())[::-1]
+        if 0 < u < x.size:
+            fragile[s] = 0 < x[u - 1] - x[u] <= 4 * err
+    keep = act & ~fragile
+    say("FP32 image error %.3g; sub-apertures with the cut inside it: %d of %d" % (err, (act & fragile).sum(),
+                                                                                  act.sum()))
+    b = c.bimg.reshape(-1, nimg, nimg)
+    np.testing.assert_allclose(b[keep], ob[keep], rtol=0, atol=max(4 * err, 1e-6))
+    assert report("brightest " + name, c.cents, oc, keep).max() < 1e-4
+    c.free()
+
+
+@pytest.mark.parametrize("ub", [10, -10, 60, -60])
+def test_brightest_noisy_ties(ub):
+    """Brightest pixels of Poisson images (integer counts: many equal values at and around the
+    cut, and below it for the subtracted one) with one count for every sub-aperture: bit for bit
+    the oracle's selection on the engine's images before selection; end to end against the FP64
+    oracle, where a Poisson count that rounds to the neighbouring step (FP32 against FP64) moves
+    the cut of its whole sub-aperture, at most a few such sub-apertures and the others exact."""
+    nsubx, n, nfft, nimg = 8, 8, 16, 8
+    say("case brightest noisy %d" % ub)
+    w = W.WFS("bn", nsubx, n, nfft, nimg, 4.2, 0.2, 0.15, 25.0)
+    c, oc, ob, b0 = _brightest_pair(nsubx, n, nfft, nimg, ub, obs=0.2, sig=150.0, addPoisson=1, phs=w.phase(),
+                                    seed=11)
+    assert _check_selection(c, b0, ub) > 0
+    act = _active(c)
+    b = c.bimg.reshape(-1, nimg, nimg)[act]
+    bad = (np.abs(b - ob[act]) > 1e-3 * np.maximum(1, np.abs(ob[act]))).any(axis=(1, 2))
+    say("sub-apertures differing from the FP64 oracle: %d of %d" % (bad.sum(), act.sum()))
+    assert bad.sum() <= 2
+    d = np.abs(c.cents.reshape(-1, 2)[act] - oc[act])
+    assert d[~bad].max() < 1e-4
+    c.free()
+
+
 def test_correlation_parabolic_fit():
     """Nyquist-sampled n=32/nfft=64/nimg=16 WFS, FFT correlation with the flat-phase image as
     the reference (util/centroid.py:1580-1626), 5x5 parabolic fit."""
