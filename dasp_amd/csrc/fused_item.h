@@ -130,6 +130,20 @@ __device__ __forceinline__ void load_item(const FusedArgs &a, const ItemRef &it,
         c.pu[i] = pu[i * NP + j];
       });
     }
+  } else if constexpr (UF && NP <= 16) {
+    // scalar row bases (clamped rows ic are wave-uniform) and one lane offset per operand, as
+    // above: nothing per row to hold across items but the uniform row index (for NP = 32 the
+    // 32 hoisted row bases cost more in scalar registers than the per-lane form)
+    const int jc = j < n ? j : n - 1;
+    const float *base = a.phs + (long)fw * a.phs_stride;
+    const unsigned po = (unsigned)(it.off + jc) * 4u, uo = (unsigned)(it.s * n * n + jc) * 4u;
+    const float jm = j < n ? 1.f : 0.f;
+    static_for<0, NP>([&](auto ii) {
+      constexpr int i = decltype(ii)::value;
+      const int ic = i < n ? i : n - 1;
+      c.ph[i] = *(const float *)((const char *)(base + (long)ic * a.npup) + po);
+      c.pu[i] = *(const float *)((const char *)(a.pup + ic * n) + uo) * (i < n ? jm : 0.f);
+    });
   } else {
     const int jc = j < n ? j : n - 1;
     static_for<0, NP>([&](auto ii) {
