@@ -73,8 +73,15 @@ __device__ __forceinline__ ItemRef flat_item(const FusedArgs &a, long first, int
   else return item_flat(a, first, sl);
 }
 
+// As in the single-WFS kernel: the NP <= 16 shapes without a spot kernel load each group when the
+// wave starts it and run three waves per SIMD; the spot shapes (two waves by their LDS) and NP = 32
+// load the next group while they transform the current one.
+template <int NP, bool SPOT>
+constexpr bool kBatchPrefetch = SPOT || NP >= 32;
+
 template <int NP, int F, int B, int WAVES, bool FAST, bool SPOT>
-__global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(NP <= 16 ? 2 : 1)))
+__global__ __launch_bounds__(64 * WAVES)
+__attribute__((amdgpu_waves_per_eu(NP <= 16 ? (kBatchPrefetch<NP, SPOT> ? 2 : 3) : 1)))
 void batch_kernel(BatchArgs b, int chunk) {
   constexpr int SPW = 64 / NP;
   __shared__ cf stage[WAVES][wave_lds_cf<NP, F, B, SPOT>()];
@@ -85,7 +92,23 @@ void batch_kernel(BatchArgs b, int chunk) {
   cf *wl = &stage[w][0];
   WaveQueue q{b.counters, (long)gridDim.x * WAVES, chunk};
   long g = q.first((long)blockIdx.x * WAVES + w);
-  if (g < b.total_groups) {
+  if (g < b.total_groups && !kBatchPrefetch<NP, SPOT>) {
+    int d = desc_of(b, g);
+    long end = desc_end(b, d);
+    while (true) {
+      PhaseCol<NP> cur;
+      const ItemRef it = flat_item<SPOT>(desc_args(b, d), (g - b.desc[d].group0) * SPW, sl);
+      load_item<NP>(desc_args(b, d), it, j, cur);
+      fused_item<NP, F, B, FAST, SPOT>(desc_args(b, d), it, j, w, sl, cur, wl);
+      const long gn = q.next(g);   // entries only grow: gn > g
+      if (gn >= b.total_groups) break;
+      if (gn >= end) {
+        d = desc_of(b, gn);
+        end = desc_end(b, d);
+      }
+      g = gn;
+    }
+  } else if (g < b.total_groups) {
     int d = desc_of(b, g);
     long base = b.desc[d].group0, end = desc_end(b, d);
     PhaseCol<NP> cur;
