@@ -30,8 +30,12 @@ constexpr int kXcds = 8;
 
 // COMP: the three-product scheme; without it the single-product FP16 form (as in the pipelined
 // kernel), which here carries the general detector options (noise, sky arrays, relative floors).
+// The single-product form loads each pair when the wave starts it and runs three waves per SIMD:
+// prefetching the next pair's operands held 16 VGPRs more and kept it at two, and the third wave
+// hides the loads better (40x40 noisy workload 0.178 -> 0.165 ms, same-box A/B). The three-product
+// forms, at two waves either way, keep the prefetch.
 template <typename T8, bool FAST, bool COMP = true>
-__global__ __launch_bounds__(256) void dft16_cog_kernel(FusedArgs a_) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(COMP ? 2 : 3))) void dft16_cog_kernel(FusedArgs a_) {
   // arguments by value (in scalar registers; some spill to vector-register lanes): reading them
   // through the argument-segment pointer, as the FP32 kernels do, measured 1-2.5 % slower here
   const FusedArgs &a = a_;
@@ -128,7 +132,7 @@ __global__ __launch_bounds__(256) void dft16_cog_kernel(FusedArgs a_) {
   // inside the loop)
   int stepF, stepR;
   pair_ref(nwaves, stepF, stepR);
-  load_pair(wave_id, cf0, cidx0, cph, cpu);
+  if constexpr (COMP) load_pair(wave_id, cf0, cidx0, cph, cpu);
   for (long pr = wave_id; pr < npairs; pr += nwaves) {
     float nph[8], npu[8];
     int nf0 = cf0 + stepF, nidx0 = cidx0 + stepR;
@@ -136,7 +140,8 @@ __global__ __launch_bounds__(256) void dft16_cog_kernel(FusedArgs a_) {
       nidx0 -= pp;
       ++nf0;
     }
-    load_pair(pr + nwaves, nf0, nidx0, nph, npu);   // next pair's loads overlap this pair's work
+    if constexpr (COMP) load_pair(pr + nwaves, nf0, nidx0, nph, npu);   // overlaps this pair's work
+    else load_pair(pr, cf0, cidx0, cph, cpu);
     float er[8], ei[8];
 #pragma unroll
     for (int t = 0; t < 8; ++t) {
@@ -147,10 +152,12 @@ __global__ __launch_bounds__(256) void dft16_cog_kernel(FusedArgs a_) {
       er[t] = cpu[t] * cs;
       ei[t] = cpu[t] * sn;
     }
+    if constexpr (COMP) {
 #pragma unroll
-    for (int t = 0; t < 8; ++t) {
-      cph[t] = nph[t];
-      cpu[t] = npu[t];
+      for (int t = 0; t < 8; ++t) {
+        cph[t] = nph[t];
+        cpu[t] = npu[t];
+      }
     }
     const int pf0 = cf0, pidx0 = cidx0;
     cf0 = nf0;
